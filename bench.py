@@ -1,0 +1,250 @@
+"""Benchmark: Gbutterfly/s and HBM roofline fraction of the MI355X NTT.
+
+Headline step: one negacyclic BabyBear transform pair (forward then inverse)
+of one n = 2^24 polynomial with uint32 coefficients resident in HBM (the
+"single large negacyclic NTT" workload).  Butterflies per transform are
+(n/2)*log2(n); algorithmic bytes per transform are 2*n*4 (one compulsory
+read and write; twiddle tables are device-resident and not counted).
+
+Extra lines in "extras": n = 2^12 cyclic latency through the uint64 ABI,
+n = 2^20 single and batched, the 24-limb RNS batch, the 512-polynomial
+uneven batch against one-polynomial-per-launch, and the uint64 ABI rate.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "ntt-tensorcore_amd"))
+sys.path.insert(0, REPO)
+
+import ntt_amd as nt  # noqa: E402
+import workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BB = W.BABYBEAR
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, warmup, stream):
+    for _ in range(warmup):
+        fn()
+    stream.sync()
+    e0, e1 = nt.Event(), nt.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    ms = e0.elapsed_ms(e1)
+    e0.destroy()
+    e1.destroy()
+    return ms
+
+
+def headline(args, ctx, stream):
+    log_n = args.log_n
+    n = 1 << log_n
+    x = W.random_poly(n, BB, 24, dtype=np.uint32)
+    buf = nt.DeviceBuffer.from_array(x)
+    fwd = nt.FLAG_NEGACYCLIC
+    inv = nt.FLAG_NEGACYCLIC | nt.FLAG_INVERSE
+
+    def step():
+        ctx.transform(buf.ptr, BB, log_n, fwd, stream)
+        ctx.transform(buf.ptr, BB, log_n, inv, stream)
+
+    ms = timed(step, args.steps, args.warmup, stream)
+    stream.sync()
+    ok = np.array_equal(buf.download(np.uint32, n), x)
+    buf.free()
+    if not ok:
+        raise RuntimeError("round trip after the timed region differs from the input")
+    ms_step = ms / args.steps
+    bfly = 2 * W.butterflies(log_n)
+    t_transform_s = ms_step / 2 / 1e3
+    achieved = 2 * n * 4 / t_transform_s / 1e9
+    return {
+        "ms_per_step": ms_step,
+        "value": bfly * args.steps / (ms / 1e3) / 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "per": "one transform (2*n*4 B, both pass-kernel launches)",
+                     "transform_us": t_transform_s * 1e6},
+    }
+
+
+def cpu_baseline(args):
+    from oracle import pyoracle as orc
+    n = 1 << args.log_n
+    t = orc.time_negacyclic_roundtrip(n, BB, 24, 1)
+    return {"value": 2 * W.butterflies(args.log_n) / t / 1e9, "unit": "Gbutterfly/s", "cores": 1,
+            "kind": "port",
+            "sample": f"1 step: negacyclic forward+inverse round trip, n=2^{args.log_n}, oracle ntt_host "
+                      f"restatement (radix-2 DIT, __int128 %), one thread, {t:.2f} s"}
+
+
+def extra_latency_4096(ctx, stream):
+    """cfg1: n=2^12 cyclic forward + inverse through the uint64 ABI, one call pair per step."""
+    n = 4096
+    cfg = nt.Config(n, BB)
+    buf = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 42))
+    lat = []
+    for i in range(60):
+        t0 = time.perf_counter()
+        cfg.forward(buf, stream=stream)
+        cfg.inverse(buf, stream=stream)
+        stream.sync()
+        if i >= 10:
+            lat.append((time.perf_counter() - t0) * 1e6)
+    ms = timed(lambda: (cfg.forward(buf, stream=stream), cfg.inverse(buf, stream=stream)), 200, 10, stream)
+    cfg.cleanup()
+    return {"p50_us_host_roundtrip": float(np.percentile(lat, 50)), "device_us_per_pair": ms * 1e3 / 200}
+
+
+def extra_2_20(ctx, stream):
+    out = {}
+    for batch in (1, 16):
+        n = 1 << 20
+        buf = nt.DeviceBuffer(n * 4 * batch)
+        for b in range(batch):
+            buf.upload(W.random_poly(n, BB, 100 + b, dtype=np.uint32), b * n * 4)
+        descs = [(buf.ptr + b * n * 4, BB, 20, nt.FLAG_NEGACYCLIC) for b in range(batch)]
+        bt = nt.Batch(ctx, descs)
+        ms = timed(lambda: bt.launch(stream), 50, 5, stream)
+        t = ms / 50 / 1e3
+        out[f"batch{batch}"] = {"Gbfly_s": batch * W.butterflies(20) / t / 1e9,
+                                "GB_s": batch * 2 * n * 4 / t / 1e9, "us": t * 1e6}
+        bt.destroy()
+        buf.free()
+    return out
+
+
+def _pack(jobs, dtype):
+    esz = np.dtype(dtype).itemsize
+    offs, tot = [], 0
+    for j in jobs:
+        offs.append(tot)
+        tot += ((1 << j["log_n"]) * esz + 255) // 256 * 256
+    buf = nt.DeviceBuffer(tot)
+    for j, o in zip(jobs, offs):
+        buf.upload(W.random_poly(1 << j["log_n"], j["q"], j["seed"], dtype=dtype), o)
+    return buf, offs
+
+
+def _flags(j, u64=False):
+    return ((nt.FLAG_INVERSE if j["inverse"] else 0) | (nt.FLAG_NEGACYCLIC if j["nega"] else 0) |
+            (nt.FLAG_U64 if u64 else 0))
+
+
+def extra_rns(ctx, stream):
+    primes = W.rns_primes()
+    jobs = [{"log_n": 16, "q": q, "inverse": False, "nega": True, "seed": 0x5EED ^ i} for i, q in enumerate(primes)]
+    buf, offs = _pack(jobs, np.uint32)
+    bt = nt.Batch(ctx, [(buf.ptr + o, j["q"], 16, _flags(j)) for j, o in zip(jobs, offs)])
+    ms = timed(lambda: bt.launch_graph(stream), 50, 5, stream)
+    t = ms / 50 / 1e3
+    bt.destroy()
+    buf.free()
+    n = 24 << 16
+    return {"limbs": 24, "log_n": 16, "us": t * 1e6, "Gbfly_s": 24 * W.butterflies(16) / t / 1e9,
+            "GB_s": 2 * n * 4 / t / 1e9, "frac": 2 * n * 4 / t / 1e9 / HBM_PEAK_GBS}
+
+
+def extra_uneven(ctx, stream):
+    jobs = W.uneven_batch()
+    buf, offs = _pack(jobs, np.uint32)
+    descs = [(buf.ptr + o, j["q"], j["log_n"], _flags(j)) for j, o in zip(jobs, offs)]
+    bt = nt.Batch(ctx, descs)
+    bt.launch_graph(stream)
+    stream.sync()
+    ms_b = timed(lambda: bt.launch_graph(stream), 20, 3, stream) / 20
+
+    def per_poly():
+        for d in descs:
+            ctx.transform(d[0], d[1], d[2], d[3], stream)
+
+    ms_p = timed(per_poly, 3, 1, stream) / 3
+    bfly = sum(W.butterflies(j["log_n"]) for j in jobs)
+    byt = sum(2 * (1 << j["log_n"]) * 4 for j in jobs)
+    info = bt.info()
+    bt.destroy()
+    buf.free()
+    return {"polys": len(jobs), "batched_us": ms_b * 1e3, "per_poly_us": ms_p * 1e3,
+            "speedup": ms_p / ms_b, "batched_Gbfly_s": bfly / (ms_b / 1e3) / 1e9,
+            "batched_GB_s": byt / (ms_b / 1e3) / 1e9, "launches": info["launches"],
+            "per_poly_launches": sum(1 if j["log_n"] <= 14 else 2 for j in jobs)}
+
+
+def extra_u64_abi(args, stream):
+    log_n = args.log_n
+    n = 1 << log_n
+    cfg = nt.Config(n, BB)
+    buf = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 24))
+    ms = timed(lambda: (cfg.forward_negacyclic(buf, stream), cfg.inverse_negacyclic(buf, stream)), 10, 2, stream)
+    cfg.cleanup()
+    buf.free()
+    t = ms / 10 / 2 / 1e3
+    return {"log_n": log_n, "Gbfly_s": W.butterflies(log_n) / t / 1e9, "GB_s": 2 * n * 8 / t / 1e9,
+            "frac": 2 * n * 8 / t / 1e9 / HBM_PEAK_GBS}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--log-n", type=int, default=24)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the NTT path runs on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+    if nt.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    stream = nt.Stream()
+    ctx = nt.Context()
+    log(f"[bench] headline: negacyclic n=2^{args.log_n} fwd+inv, {args.warmup} warmup + {args.steps} steps")
+    h = headline(args, ctx, stream)
+    log(f"[bench] headline done: {h['value']:.1f} Gbfly/s, {h['roofline']['achieved']:.0f} GB/s")
+    extras = {}
+    if not args.no_extras:
+        for name, fn in [("cyclic_4096", lambda: extra_latency_4096(ctx, stream)),
+                         ("negacyclic_2_20", lambda: extra_2_20(ctx, stream)),
+                         ("rns_24x2_16", lambda: extra_rns(ctx, stream)),
+                         ("uneven_512", lambda: extra_uneven(ctx, stream)),
+                         ("u64_abi", lambda: extra_u64_abi(args, stream))]:
+            log(f"[bench] extra {name}")
+            extras[name] = fn()
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle, one thread)")
+        cpu = cpu_baseline(args)
+    out = {
+        "metric": "Gbutterfly/s + %HBM roofline, batched vs per-poly launch; p99 latency at QPS",
+        "value": h["value"], "unit": "Gbutterfly/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": h["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic (splitmix64-seeded uniform coefficients in [0,q))",
+        "config": {"workload": f"negacyclic NTT over BabyBear q=2013265921, n=2^{args.log_n}, forward then "
+                               "inverse, one polynomial, uint32 coefficients resident in HBM",
+                   "log_n": args.log_n, "q": BB, "transform": "negacyclic", "step": "forward+inverse"},
+        "roofline": h["roofline"], "cpu_baseline": cpu, "extras": extras,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.destroy()
+    stream.destroy()
+
+
+if __name__ == "__main__":
+    main()

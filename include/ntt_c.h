@@ -1,0 +1,187 @@
+/*
+ * ntt_c.h — C ABI of libntt_mi355x.so, the MI355X (gfx950) NTT library.
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or torch types.  Every
+ * entry point returns an ntt_status (0 = success) and never exits the
+ * process.  Device pointers are HIP device memory; `stream` is a hipStream_t
+ * passed as void* (NULL = the default stream).  All transforms are
+ * asynchronous on that stream and allocate nothing on the transform path
+ * (tables and workspace are made by the init/plan calls).
+ *
+ * Each entry point names the reference interface it replaces
+ * (MMRROOO/ntt-tensorcore, file:line).  include/ntt.cuh and
+ * include/ntt_arbitrary.cuh wrap these in the reference's C++ names.
+ */
+#ifndef NTT_C_H_
+#define NTT_C_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NTT_OK = 0,
+    NTT_ERR_INVALID_ARG = 1,  /* null pointer, n not a power of two, bad variant/flags */
+    NTT_ERR_NO_ROOT = 2,      /* q prime-test failed or (2)n does not divide q-1 */
+    NTT_ERR_OOM = 3,          /* device or host allocation failed / over budget */
+    NTT_ERR_DEVICE = 4,       /* a HIP call failed */
+    NTT_ERR_UNSUPPORTED = 5,  /* size or modulus outside what this build handles */
+    NTT_ERR_BUSY = 6,         /* serving queue full / request not finished */
+} ntt_status;
+
+/* ModularParams, reference include/modular_arith.cuh:11-19 (same fields, same order). */
+typedef struct {
+    uint64_t q;
+    uint64_t q_twice;
+    uint64_t barrett_k;
+    uint64_t barrett_m;
+    uint64_t omega;      /* primitive n-th root g^((q-1)/n), g by the reference's generator rule */
+    uint64_t omega_inv;
+    uint64_t n_inv;
+} ntt_c_modular_params;
+
+/* NTTConfig, reference include/ntt.cuh:11-24.  The leading fields are the
+ * reference's; `impl` holds this library's plans and workspace. */
+typedef struct {
+    uint64_t n;
+    uint64_t log_n;
+    uint64_t q;
+    uint64_t mu;                 /* floor(2^64 / q) */
+    uint64_t* d_twiddles;        /* device: omega^k, k < n */
+    uint64_t* d_twiddles_inv;    /* device: omega^-k, k < n */
+    ntt_c_modular_params params;
+    void* impl;
+} ntt_c_config;
+
+/* NTTVariant, reference include/ntt.cuh:27-32. */
+enum { NTT_VARIANT_BASIC = 0, NTT_VARIANT_FOUR_STEP = 1, NTT_VARIANT_TLMOP = 2, NTT_VARIANT_FULL_OPTIMIZED = 3 };
+
+/* Transform flags for the native descriptor API. */
+enum {
+    NTT_FLAG_INVERSE = 1u,     /* inverse transform (includes n^-1) */
+    NTT_FLAG_NEGACYCLIC = 2u,  /* Z_q[x]/(x^n+1) with psi = g^((q-1)/2n) */
+    NTT_FLAG_U64 = 4u,         /* coefficients stored as uint64 (else uint32; q >= 2^31 requires U64) */
+};
+
+/* ---------------- 31-bit path: include/ntt.cuh ---------------- */
+
+/* ntt_init, reference src/ntt_basic.cu:217-249 (include/ntt.cuh:35).  Builds
+ * params by the reference's generator rule, the device twiddle tables and the
+ * transform plans.  q < 2^62 prime with n | q-1. */
+ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out);
+/* ntt_cleanup, reference src/ntt_basic.cu:251-257 (include/ntt.cuh:38). */
+ntt_status ntt_c_cleanup(ntt_c_config* cfg);
+
+/* ntt_forward_{basic,four_step,tlmop,optimized} / ntt_inverse_*, reference
+ * include/ntt.cuh:42-57 (src/ntt_basic.cu:268-334, src/ntt_four_step.cu:244-392,
+ * src/ntt_optimized.cu:515-613).  In place, natural order in and out,
+ * canonical values; every variant computes ntt_host's transform. */
+ntt_status ntt_c_forward(uint64_t* d_data, const ntt_c_config* cfg, int variant, void* stream);
+ntt_status ntt_c_inverse(uint64_t* d_data, const ntt_c_config* cfg, int variant, void* stream);
+
+/* Negacyclic transform over Z_q[x]/(x^n+1): A_k = sum_j a_j psi^(j(2k+1)).
+ * The reference names it (README.md:30) but has no code
+ * (tests/test_ntt.cu:304-307); needs 2n | q-1. */
+ntt_status ntt_c_forward_negacyclic(uint64_t* d_data, const ntt_c_config* cfg, void* stream);
+ntt_status ntt_c_inverse_negacyclic(uint64_t* d_data, const ntt_c_config* cfg, void* stream);
+
+/* ntt_forward_batch / ntt_inverse_batch, reference src/ntt_basic.cu:337-389
+ * (include/ntt.cuh:70-75): batch_size polynomials at d_data + i*n, all in one
+ * launch sequence instead of the reference's per-polynomial loop. */
+ntt_status ntt_c_forward_batch(uint64_t* d_data, uint64_t batch_size, const ntt_c_config* cfg, int variant,
+                               void* stream);
+ntt_status ntt_c_inverse_batch(uint64_t* d_data, uint64_t batch_size, const ntt_c_config* cfg, int variant,
+                               void* stream);
+
+/* poly_multiply_ntt, reference src/ntt_basic.cu:392-409 (include/ntt.cuh:78-79). */
+ntt_status ntt_c_poly_multiply(uint64_t* d_result, const uint64_t* d_a, const uint64_t* d_b,
+                               const ntt_c_config* cfg, void* stream);
+
+/* Negacyclic product a*b mod (x^n+1, q) into d_result; d_a, d_b are
+ * overwritten with their forward transforms (the pipeline of
+ * tests/test_ntt.cu:177-240 on the negacyclic transform). */
+ntt_status ntt_c_negacyclic_multiply(uint64_t* d_result, uint64_t* d_a, uint64_t* d_b, const ntt_c_config* cfg,
+                                     void* stream);
+
+/* bit_reverse_permute, reference src/ntt_basic.cu:259-265 (include/ntt.cuh:82). */
+ntt_status ntt_c_bit_reverse(uint64_t* d_data, uint64_t n, void* stream);
+
+/* ntt_forward_rowmaj, reference src/ntt_optimized.cu:940-965 (include/ntt.cuh:63):
+ * input and output in the pre-transposed layout data[b*N1 + a] = canon[a*N2 + b]
+ * with N1 = 2^ceil(log n / 2), N2 = n / N1 (N1 = N2 = 64 at n = 4096, as in the
+ * reference).  Any power-of-two n. */
+ntt_status ntt_c_forward_rowmaj(uint64_t* d_data, const ntt_c_config* cfg, void* stream);
+/* rowmaj_to_pretransposed / rowmaj_from_pretransposed, reference
+ * src/ntt_optimized.cu:922-934 (include/ntt.cuh:64-67). */
+ntt_status ntt_c_rowmaj_to_pretransposed(uint64_t* d_dst, const uint64_t* d_src, int n1, int n2, void* stream);
+ntt_status ntt_c_rowmaj_from_pretransposed(uint64_t* d_dst, const uint64_t* d_src, int n1, int n2, void* stream);
+
+/* ntt_host, reference src/ntt_basic.cu:412-463 (include/ntt.cuh:85): the host
+ * transform of the public API (radix-2 DIT on the CPU, natural order). */
+ntt_status ntt_c_host(uint64_t* data, uint64_t n, const ntt_c_modular_params* params, int inverse);
+
+/* create_params / find_primitive_root, reference include/modular_arith.cuh:130-170. */
+ntt_status ntt_c_create_params(uint64_t q, uint64_t n, ntt_c_modular_params* out);
+uint64_t ntt_c_find_primitive_root(uint64_t q, uint64_t n);
+
+/* ---------------- native descriptor API (no reference counterpart: the
+ * reference's only batching is the per-polynomial loop above) ---------------- */
+
+typedef struct {
+    void* data;       /* device pointer, n coefficients (uint32 or uint64 per NTT_FLAG_U64), in place */
+    uint64_t q;       /* prime, q < 2^31 with uint32 storage; q < 2^62 with NTT_FLAG_U64 */
+    uint32_t log_n;   /* 1 .. 26 */
+    uint32_t flags;   /* NTT_FLAG_* */
+} ntt_c_desc;
+
+typedef struct ntt_c_context ntt_c_context;  /* device-resident plan/twiddle cache */
+typedef struct ntt_c_batch ntt_c_batch;      /* a prepared batch (job tables, workspace, graph) */
+
+ntt_status ntt_c_context_create(ntt_c_context** out);
+ntt_status ntt_c_context_destroy(ntt_c_context* ctx);
+/* Bytes of device memory the context's tables use. */
+uint64_t ntt_c_context_table_bytes(const ntt_c_context* ctx);
+
+/* One transform, one polynomial, launched on its own (the per-polynomial path). */
+ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* desc, void* stream);
+
+/* Prepare a batch of descriptors (mixed n, q, direction, transform type):
+ * builds tables for every (q, n) in the batch, the device job lists and the
+ * workspace.  The descriptors' data pointers are bound at prepare time. */
+ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count, ntt_c_batch** out);
+/* Launch a prepared batch: at most three kernel launches whatever the mix
+ * (single-pass polynomials, then the two passes of the four-step ones). */
+ntt_status ntt_c_batch_launch(ntt_c_batch* batch, void* stream);
+/* Capture the batch's launches into a hipGraph once; later calls replay it. */
+ntt_status ntt_c_batch_launch_graph(ntt_c_batch* batch, void* stream);
+ntt_status ntt_c_batch_destroy(ntt_c_batch* batch);
+/* Launch counts and workspace of a prepared batch. */
+ntt_status ntt_c_batch_info(const ntt_c_batch* batch, uint32_t* launches, uint64_t* workspace_bytes,
+                            uint64_t* workgroups);
+
+/* ---------------- device-memory and timing helpers for hosts without a HIP
+ * binding (tests, benchmarks, FFI callers) ---------------- */
+ntt_status ntt_c_device_count(int* count);
+ntt_status ntt_c_malloc(void** ptr, uint64_t bytes);
+ntt_status ntt_c_free(void* ptr);
+ntt_status ntt_c_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
+ntt_status ntt_c_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+ntt_status ntt_c_memset(void* dst, int value, uint64_t bytes);
+ntt_status ntt_c_stream_create(void** stream);
+ntt_status ntt_c_stream_destroy(void* stream);
+ntt_status ntt_c_stream_sync(void* stream);
+ntt_status ntt_c_device_sync(void);
+ntt_status ntt_c_event_create(void** ev);
+ntt_status ntt_c_event_destroy(void* ev);
+ntt_status ntt_c_event_record(void* ev, void* stream);
+ntt_status ntt_c_event_elapsed_ms(float* ms, void* start, void* end);
+/* Human-readable text of the last error on this thread. */
+const char* ntt_c_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NTT_C_H_ */

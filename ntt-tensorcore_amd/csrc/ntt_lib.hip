@@ -1,0 +1,956 @@
+// ntt_lib.hip — plans, launches and the C ABI of libntt_mi355x.so.
+//
+// A Plan holds the device-resident tables for one (q, log n, cyclic or
+// negacyclic, forward or inverse) transform and the pass descriptions that
+// use them.  Every public transform — the reference's single-polynomial
+// entry points, its batch loop, and this library's descriptor batches — is a
+// list of PassJobs fed to one kernel template (ntt_kernels.hpp).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/ntt_c.h"
+#include "field_host.hpp"
+#include "ntt_kernels.hpp"
+
+using namespace nttk;
+using namespace ntth;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+
+static ntt_status fail(ntt_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(NTT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+extern "C" const char* ntt_c_last_error(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------ device kernels (small)
+namespace nttk {
+
+__global__ void pointwise_mul62_kernel(uint64_t* r, const uint64_t* a, const uint64_t* b, uint64_t n, uint64_t q,
+                                       uint64_t qneg_inv, uint64_t r2) {
+    // Montgomery with R = 2^64: mont(x,y) = x*y*R^-1 mod q; mont(mont(a,b), R^2) = a*b.
+    auto mont = [&](uint64_t x, uint64_t y) -> uint64_t {
+        const uint64_t lo = x * y, hi = __umul64hi(x, y);
+        const uint64_t m = lo * qneg_inv;
+        const uint64_t t = hi + __umul64hi(m, q) + (lo != 0);
+        return t >= q ? t - q : t;
+    };
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) r[i] = mont(mont(a[i], b[i]), r2);
+}
+
+__global__ void bitrev_kernel(uint64_t* d, uint64_t n, uint32_t log_n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const uint64_t r = (uint64_t)bitrev_bits((uint32_t)i, log_n);
+        if (i < r) {
+            const uint64_t t = d[i];
+            d[i] = d[r];
+            d[r] = t;
+        }
+    }
+}
+
+// out[b*n1 + a] = in[a*n2 + b] for a < n1, b < n2 (a 32x32 LDS tile per workgroup).
+__global__ void transpose_u64_kernel(uint64_t* out, const uint64_t* in, uint32_t n1, uint32_t n2) {
+    __shared__ uint64_t t[32][33];
+    const uint32_t a0 = blockIdx.y * 32, b0 = blockIdx.x * 32;
+    for (uint32_t i = threadIdx.y; i < 32; i += blockDim.y) {
+        const uint32_t a = a0 + i, b = b0 + threadIdx.x;
+        if (a < n1 && b < n2) t[i][threadIdx.x] = in[(uint64_t)a * n2 + b];
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.y; i < 32; i += blockDim.y) {
+        const uint32_t b = b0 + i, a = a0 + threadIdx.x;
+        if (a < n1 && b < n2) out[(uint64_t)b * n1 + a] = t[threadIdx.x][i];
+    }
+}
+
+// out[k] = lo[k & mask] * hi[k >> bits] (Shoup on hi): powers omega^k from two tables.
+template <class M>
+__global__ void powers_kernel(uint64_t* out, uint64_t n, const TwP<typename M::V>* lo, const TwP<typename M::V>* hi,
+                              uint32_t bits, uint64_t q) {
+    const M mod{(decltype(M::q))q};
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = mod.mul(lo[i & ((1ull << bits) - 1)].w, hi[i >> bits]);
+}
+
+}  // namespace nttk
+
+// ------------------------------------------------------------------ device limits
+static int g_max_lds = -1;
+
+static int device_max_lds() {
+    if (g_max_lds < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            g_max_lds = v;
+        else
+            g_max_lds = 65536;
+    }
+    return g_max_lds;
+}
+
+// Tile budget (elements of the arithmetic type) for one workgroup's LDS tile.
+static const uint32_t kLdsBytes = 65536;
+
+// ------------------------------------------------------------------ plans
+enum class Arith { M32, M64 };
+
+struct PassSpec {
+    uint32_t log_r = 0, log_c = 0;
+    int z_off = -1;          // byte offset of the CT table in the plan's block
+    uint32_t post = POST_NONE;
+    int lo_off = -1, hi_off = -1;
+    uint32_t e_a = 0, e_b = 0, e_d = 0, e_mask = 0, lo_bits = 0;
+    uint64_t scale_w = 0, scale_wp = 0;
+};
+
+struct Plan {
+    uint64_t q = 0;
+    uint32_t log_n = 0;
+    bool nega = false, inv = false;
+    Arith arith = Arith::M32;
+    int npass = 1;
+    PassSpec pass[2];
+    uint32_t log_n1 = 0, log_n2 = 0;  // two-pass split
+    void* block = nullptr;            // device tables
+    uint64_t bytes = 0;
+    ~Plan() {
+        if (block) (void)hipFree(block);
+    }
+};
+
+static uint32_t vbytes(Arith a) { return a == Arith::M32 ? 4u : 8u; }
+
+static uint32_t tile_log_elems(Arith a) {
+    uint32_t e = kLdsBytes / vbytes(a);
+    uint32_t l = 0;
+    while ((2u << l) <= e) l++;
+    return l;  // 14 for 32-bit values, 13 for 64-bit
+}
+
+// Validates (q, n) the way the transforms need: prime q, 2^log_n | q-1
+// (2^(log_n+1) for the negacyclic root).
+static ntt_status check_modulus(uint64_t q, uint32_t log_n, bool nega) {
+    if (q < 3 || q >= (1ull << 62)) return fail(NTT_ERR_UNSUPPORTED, "modulus must satisfy 3 <= q < 2^62");
+    if (!is_prime(q)) return fail(NTT_ERR_NO_ROOT, "modulus is not prime");
+    const uint32_t need = log_n + (nega ? 1u : 0u);
+    if (need >= 63 || ((q - 1) & ((1ull << need) - 1)) != 0)
+        return fail(NTT_ERR_NO_ROOT, nega ? "2n does not divide q-1" : "n does not divide q-1");
+    return NTT_OK;
+}
+
+struct TableBuilder {
+    Arith arith;
+    uint64_t q;
+    std::vector<unsigned char> host;
+    int add(const std::vector<uint64_t>& vals) {
+        // Stored as Shoup pairs of the arithmetic width, 16-byte aligned.
+        size_t off = (host.size() + 15) & ~size_t(15);
+        const int bits = arith == Arith::M32 ? 32 : 64;
+        const size_t esz = arith == Arith::M32 ? 8 : 16;
+        host.resize(off + vals.size() * esz);
+        for (size_t i = 0; i < vals.size(); i++) {
+            const uint64_t w = vals[i], wp = shoup(w, q, bits);
+            if (arith == Arith::M32) {
+                uint32_t p[2] = {(uint32_t)w, (uint32_t)wp};
+                memcpy(&host[off + i * esz], p, 8);
+            } else {
+                uint64_t p[2] = {w, wp};
+                memcpy(&host[off + i * esz], p, 16);
+            }
+        }
+        return (int)off;
+    }
+};
+
+// Builds the tables of one transform.  The formulas are the decompositions
+// checked against ntt_host (see the header of ntt_kernels.hpp):
+//   single pass        : CT table over N (root omega, factor psi if negacyclic);
+//                        inverse cyclic: scale n^-1; inverse negacyclic: row table psi^-k n^-1.
+//   two pass N = N1*N2 : pass 1 CT over N1 (root omega^N2, factor psi^N2 if forward
+//                        negacyclic) then four-step twiddle; pass 2 CT over N2 (root
+//                        omega^N1), then psi^(-N1 k) for the inverse negacyclic.
+static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Arith arith, std::unique_ptr<Plan>& out) {
+    ntt_status st = check_modulus(q, log_n, nega);
+    if (st != NTT_OK) return st;
+    if (arith == Arith::M32 && q >= (1ull << 31)) return fail(NTT_ERR_UNSUPPORTED, "32-bit arithmetic needs q < 2^31");
+    if (log_n < 1 || log_n > 26) return fail(NTT_ERR_UNSUPPORTED, "log2 n must be in [1, 26]");
+    auto P = std::make_unique<Plan>();
+    P->q = q;
+    P->log_n = log_n;
+    P->nega = nega;
+    P->inv = inv;
+    P->arith = arith;
+    const uint64_t N = 1ull << log_n;
+    const uint64_t g = generator(q);
+    const uint64_t omega = powmod(g, (q - 1) / N, q);
+    const uint64_t psi = nega ? powmod(g, (q - 1) / (2 * N), q) : 1;
+    const uint64_t w = inv ? invmod(omega, q) : omega;
+    const uint64_t p = inv ? invmod(psi, q) : psi;
+    const uint64_t ninv = invmod(N % q, q);
+    TableBuilder tb{arith, q, {}};
+    const uint32_t tl = tile_log_elems(arith);
+
+    if (log_n <= tl) {
+        P->npass = 1;
+        PassSpec& s = P->pass[0];
+        s.log_r = log_n;
+        s.log_c = tl - log_n;  // several polynomials per tile when batched
+        s.z_off = tb.add(ct_table(log_n, w, (nega && !inv) ? p : 1, q));
+        if (inv && !nega) {
+            s.post = POST_SCALE;
+            s.scale_w = ninv;
+            s.scale_wp = shoup(ninv, q, arith == Arith::M32 ? 32 : 64);
+        } else if (inv && nega) {
+            s.post = POST_ROW;
+            s.lo_off = tb.add(powers(p, N, q, ninv));
+        }
+    } else {
+        P->npass = 2;
+        const uint32_t l1 = (log_n + 1) / 2, l2 = log_n / 2;
+        P->log_n1 = l1;
+        P->log_n2 = l2;
+        const uint64_t N1 = 1ull << l1, N2 = 1ull << l2;
+        PassSpec& a = P->pass[0];
+        a.log_r = l1;
+        a.log_c = std::min(tl - l1, l2);
+        a.z_off = tb.add(ct_table(l1, powmod(w, N2, q), (nega && !inv) ? powmod(p, N2, q) : 1, q));
+        a.post = POST_FOURSTEP;
+        // exponent modulus M = N (cyclic) or 2N (negacyclic), split in two tables
+        const uint32_t mbits = log_n + (nega ? 1 : 0);
+        const uint32_t lob = (mbits + 1) / 2;
+        const uint64_t base = nega ? p : w;
+        a.e_mask = (uint32_t)((1ull << mbits) - 1);
+        a.lo_bits = lob;
+        if (!nega) { a.e_a = 1; a.e_b = 0; a.e_d = 0; }        // omega^(k c)
+        else if (!inv) { a.e_a = 2; a.e_b = 0; a.e_d = 1; }    // psi^(c (2k+1))
+        else { a.e_a = 2; a.e_b = 1; a.e_d = 0; }              // psi^-(k (2c+1))
+        a.lo_off = tb.add(powers(base, 1ull << lob, q));
+        a.hi_off = tb.add(powers(powmod(base, 1ull << lob, q), 1ull << (mbits - lob), q, inv ? ninv : 1));
+        PassSpec& b = P->pass[1];
+        b.log_r = l2;
+        b.log_c = std::min(tl - l2, l1);
+        b.z_off = tb.add(ct_table(l2, powmod(w, N1, q), 1, q));
+        if (nega && inv) {
+            b.post = POST_ROW;
+            b.lo_off = tb.add(powers(powmod(p, N1, q), N2, q));
+        }
+    }
+    P->bytes = tb.host.size();
+    hipError_t e = hipMalloc(&P->block, P->bytes);
+    if (e != hipSuccess) return fail(NTT_ERR_OOM, std::string("hipMalloc(tables): ") + hipGetErrorString(e));
+    e = hipMemcpy(P->block, tb.host.data(), P->bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(NTT_ERR_DEVICE, std::string("hipMemcpy(tables): ") + hipGetErrorString(e));
+    out = std::move(P);
+    return NTT_OK;
+}
+
+// Scratch elements (arithmetic width) one polynomial needs for this plan.
+static uint64_t plan_scratch_elems(const Plan& P) { return P.npass == 2 ? (1ull << P.log_n) : 0; }
+
+enum class Store { U32, U64 };
+
+// Fills the PassJobs of `npoly` polynomials of plan P at data (stride n
+// elements), with four-step scratch at `scratch` (stride n, arithmetic width).
+static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, PassJob out[2]) {
+    const uint64_t N = 1ull << P.log_n;
+    const char* blk = (const char*)P.block;
+    for (int i = 0; i < P.npass; i++) {
+        const PassSpec& s = P.pass[i];
+        PassJob& J = out[i];
+        memset(&J, 0, sizeof(J));
+        J.z = blk + s.z_off;
+        J.post_lo = s.lo_off >= 0 ? blk + s.lo_off : nullptr;
+        J.post_hi = s.hi_off >= 0 ? blk + s.hi_off : nullptr;
+        J.q = P.q;
+        J.scale_w = s.scale_w;
+        J.scale_wp = s.scale_wp;
+        J.log_r = s.log_r;
+        J.post = s.post;
+        J.e_a = s.e_a;
+        J.e_b = s.e_b;
+        J.e_d = s.e_d;
+        J.e_mask = s.e_mask;
+        J.lo_bits = s.lo_bits;
+        if (P.npass == 1) {
+            // columns are polynomials: element (r, p) at p*N + r
+            J.in = data;
+            J.out = data;
+            J.s_r = 1; J.s_c = (int64_t)N;
+            J.o_r = 1; J.o_c = (int64_t)N;
+            const uint32_t lc = (uint32_t)std::min<uint64_t>(s.log_c, 63 - __builtin_clzll(npoly | 1));
+            J.log_c = (npoly >= (1ull << s.log_c)) ? s.log_c : lc;
+            J.ncols = (uint32_t)npoly;
+            J.tiles_per_poly = (uint32_t)((npoly + (1ull << J.log_c) - 1) >> J.log_c);
+            J.npoly = 1;
+            J.in_pstride = J.out_pstride = 0;
+        } else {
+            const uint64_t N1 = 1ull << P.log_n1, N2 = 1ull << P.log_n2;
+            J.log_c = s.log_c;
+            J.npoly = (uint32_t)npoly;
+            if (i == 0) {  // data [N1][N2] -> scratch [N2][N1] (transposed)
+                J.in = data; J.out = scratch;
+                J.s_r = (int64_t)N2; J.s_c = 1;
+                J.o_r = 1; J.o_c = (int64_t)N1;
+                J.ncols = (uint32_t)N2;
+            } else {       // scratch [N2][N1] -> data (k2, k1) at k2*N1 + k1
+                J.in = scratch; J.out = data;
+                J.s_r = (int64_t)N1; J.s_c = 1;
+                J.o_r = (int64_t)N1; J.o_c = 1;
+                J.ncols = (uint32_t)N1;
+            }
+            J.tiles_per_poly = J.ncols >> J.log_c;
+            J.in_pstride = J.out_pstride = (int64_t)N;
+        }
+    }
+    (void)st;
+}
+
+static uint32_t job_tiles(const PassJob& J) { return J.tiles_per_poly * (J.npoly ? J.npoly : 1); }
+
+template <class M, class Sin, class Sout>
+static ntt_status launch_t(const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t nblocks, uint32_t lds,
+                           hipStream_t stream) {
+    auto fn = pass_kernel<M, Sin, Sout, 4>;
+    if (lds > 65536) {
+        static std::once_flag once;
+        std::call_once(once, [&] {
+            (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, device_max_lds());
+        });
+    }
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(256), lds, stream, djobs, njobs, inl);
+    HIP_TRY(hipGetLastError());
+    return NTT_OK;
+}
+
+// One launch of the pass kernel for (arith, input storage, output storage).
+static ntt_status launch_pass(Arith ar, Store sin, Store sout, const PassJob* djobs, uint32_t njobs,
+                              const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream) {
+    if (nblocks == 0) return NTT_OK;
+    if (ar == Arith::M32) {
+        if (sin == Store::U32 && sout == Store::U32) return launch_t<Mod32, uint32_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
+        if (sin == Store::U64 && sout == Store::U32) return launch_t<Mod32, uint64_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
+        if (sin == Store::U32 && sout == Store::U64) return launch_t<Mod32, uint32_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
+        return launch_t<Mod32, uint64_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
+    }
+    return launch_t<Mod64, uint64_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
+}
+
+static uint32_t job_lds(const PassJob& J, Arith ar) { return (1u << (J.log_r + J.log_c)) * vbytes(ar); }
+
+// Runs plan P on npoly polynomials at data (contiguous, stride n).
+static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, hipStream_t stream) {
+    PassJob jobs[2];
+    make_jobs(P, data, st, npoly, scratch, jobs);
+    const Store mid = P.arith == Arith::M32 ? Store::U32 : Store::U64;
+    if (P.npass == 1) return launch_pass(P.arith, st, st, nullptr, 0, jobs[0], job_tiles(jobs[0]), job_lds(jobs[0], P.arith), stream);
+    ntt_status s = launch_pass(P.arith, st, mid, nullptr, 0, jobs[0], job_tiles(jobs[0]), job_lds(jobs[0], P.arith), stream);
+    if (s != NTT_OK) return s;
+    return launch_pass(P.arith, mid, st, nullptr, 0, jobs[1], job_tiles(jobs[1]), job_lds(jobs[1], P.arith), stream);
+}
+
+// ------------------------------------------------------------------ config (include/ntt.cuh)
+struct ConfigImpl {
+    std::unique_ptr<Plan> plans[4];  // [nega*2 + inv]
+    ntt_status plan_status[4] = {NTT_OK, NTT_OK, NTT_OK, NTT_OK};
+    std::string plan_error[4];
+    Arith arith = Arith::M32;
+    std::mutex mu;
+    std::map<hipStream_t, std::pair<void*, uint64_t>> scratch;  // per stream: ptr, bytes
+    ~ConfigImpl() {
+        for (auto& kv : scratch)
+            if (kv.second.first) (void)hipFree(kv.second.first);
+    }
+};
+
+// Workspace for two-pass transforms, one buffer per stream so that one config
+// can serve several streams at once; grown on demand, never shrunk.
+static ntt_status scratch_for(ConfigImpl* I, hipStream_t stream, uint64_t bytes, void** out) {
+    *out = nullptr;
+    if (bytes == 0) return NTT_OK;
+    std::lock_guard<std::mutex> lk(I->mu);
+    auto& ent = I->scratch[stream];
+    if (ent.second < bytes) {
+        if (ent.first) {
+            HIP_TRY(hipStreamSynchronize(stream));
+            HIP_TRY(hipFree(ent.first));
+            ent.first = nullptr;
+            ent.second = 0;
+        }
+        hipError_t e = hipMalloc(&ent.first, bytes);
+        if (e != hipSuccess) return fail(NTT_ERR_OOM, "hipMalloc(scratch) failed");
+        ent.second = bytes;
+    }
+    *out = ent.first;
+    return NTT_OK;
+}
+
+extern "C" uint64_t ntt_c_find_primitive_root(uint64_t q, uint64_t n) {
+    if (q < 3 || n == 0) return 0;
+    return powmod(generator(q), (q - 1) / n, q);
+}
+
+extern "C" ntt_status ntt_c_create_params(uint64_t q, uint64_t n, ntt_c_modular_params* p) {
+    if (!p || q < 3 || n == 0 || (n & (n - 1))) return fail(NTT_ERR_INVALID_ARG, "create_params: bad q or n");
+    // create_params, reference include/modular_arith.cuh:153-170
+    p->q = q;
+    p->q_twice = 2 * q;
+    int k = 64;
+    while ((1ull << (k - 64)) < q && k < 96) k++;
+    p->barrett_k = (uint64_t)k;
+    p->barrett_m = (uint64_t)(((u128)1 << k) / q);
+    p->omega = ntt_c_find_primitive_root(q, n);
+    p->omega_inv = invmod(p->omega, q);
+    p->n_inv = invmod(n % q, q);
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out) {
+    if (!out) return fail(NTT_ERR_INVALID_ARG, "ntt_init: null output");
+    *out = nullptr;
+    if (n < 2 || (n & (n - 1))) return fail(NTT_ERR_INVALID_ARG, "ntt_init: n must be a power of two >= 2");
+    const uint32_t log_n = (uint32_t)__builtin_ctzll(n);
+    ntt_status st = check_modulus(q, log_n, false);
+    if (st != NTT_OK) return st;
+    if (log_n > 26) return fail(NTT_ERR_UNSUPPORTED, "ntt_init: n > 2^26");
+    auto C = std::make_unique<ntt_c_config>();
+    memset(C.get(), 0, sizeof(ntt_c_config));
+    C->n = n;
+    C->log_n = log_n;
+    C->q = q;
+    C->mu = (uint64_t)(((u128)1 << 64) / q);
+    ntt_c_create_params(q, n, &C->params);
+    auto I = std::make_unique<ConfigImpl>();
+    I->arith = q < (1ull << 31) ? Arith::M32 : Arith::M64;
+    for (int nega = 0; nega < 2; nega++)
+        for (int inv = 0; inv < 2; inv++) {
+            const int k = nega * 2 + inv;
+            I->plan_status[k] = build_plan(q, log_n, nega, inv, I->arith, I->plans[k]);
+            if (I->plan_status[k] != NTT_OK) {
+                I->plan_error[k] = g_last_error;
+                if (!nega) return I->plan_status[k];  // cyclic must exist
+            }
+        }
+    // d_twiddles[k] = omega^k, d_twiddles_inv[k] = omega^-k (reference src/ntt_basic.cu:233-246),
+    // generated on the device from two sqrt(n)-entry tables.
+    HIP_TRY(hipMalloc(&C->d_twiddles, n * sizeof(uint64_t)));
+    HIP_TRY(hipMalloc(&C->d_twiddles_inv, n * sizeof(uint64_t)));
+    for (int inv = 0; inv < 2; inv++) {
+        const uint64_t w = inv ? C->params.omega_inv : C->params.omega;
+        const uint32_t bits = (log_n + 1) / 2;
+        TableBuilder tb{I->arith, q, {}};
+        const int lo = tb.add(powers(w, 1ull << bits, q));
+        const int hi = tb.add(powers(powmod(w, 1ull << bits, q), (n >> bits) + 1, q));
+        void* dt = nullptr;
+        HIP_TRY(hipMalloc(&dt, tb.host.size()));
+        HIP_TRY(hipMemcpy(dt, tb.host.data(), tb.host.size(), hipMemcpyHostToDevice));
+        uint64_t* dst = inv ? C->d_twiddles_inv : C->d_twiddles;
+        const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 4096);
+        if (I->arith == Arith::M32)
+            hipLaunchKernelGGL(powers_kernel<Mod32>, dim3(blocks), dim3(256), 0, 0, dst, n,
+                               (const TwP<uint32_t>*)((char*)dt + lo), (const TwP<uint32_t>*)((char*)dt + hi), bits, q);
+        else
+            hipLaunchKernelGGL(powers_kernel<Mod64>, dim3(blocks), dim3(256), 0, 0, dst, n,
+                               (const TwP<uint64_t>*)((char*)dt + lo), (const TwP<uint64_t>*)((char*)dt + hi), bits, q);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipFree(dt));
+    }
+    C->impl = I.release();
+    *out = C.release();
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_cleanup(ntt_c_config* cfg) {
+    if (!cfg) return NTT_OK;
+    (void)hipDeviceSynchronize();
+    if (cfg->d_twiddles) (void)hipFree(cfg->d_twiddles);
+    if (cfg->d_twiddles_inv) (void)hipFree(cfg->d_twiddles_inv);
+    delete (ConfigImpl*)cfg->impl;
+    delete cfg;
+    return NTT_OK;
+}
+
+static ntt_status run_config(uint64_t* d, const ntt_c_config* cfg, bool nega, bool inv, uint64_t batch,
+                             void* stream) {
+    if (!d || !cfg || !cfg->impl) return fail(NTT_ERR_INVALID_ARG, "null data or config");
+    if (batch == 0) return NTT_OK;
+    ConfigImpl* I = (ConfigImpl*)cfg->impl;
+    const int k = (nega ? 2 : 0) + (inv ? 1 : 0);
+    if (!I->plans[k]) return fail(I->plan_status[k], I->plan_error[k]);
+    const Plan& P = *I->plans[k];
+    void* scratch = nullptr;
+    const uint64_t sbytes = plan_scratch_elems(P) * batch * vbytes(P.arith);
+    ntt_status st = scratch_for(I, (hipStream_t)stream, sbytes, &scratch);
+    if (st != NTT_OK) return st;
+    return run_plan(P, d, Store::U64, batch, scratch, (hipStream_t)stream);
+}
+
+static bool valid_variant(int v) { return v >= NTT_VARIANT_BASIC && v <= NTT_VARIANT_FULL_OPTIMIZED; }
+
+extern "C" ntt_status ntt_c_forward(uint64_t* d, const ntt_c_config* cfg, int variant, void* stream) {
+    if (!valid_variant(variant)) return fail(NTT_ERR_INVALID_ARG, "bad variant");
+    return run_config(d, cfg, false, false, 1, stream);
+}
+extern "C" ntt_status ntt_c_inverse(uint64_t* d, const ntt_c_config* cfg, int variant, void* stream) {
+    if (!valid_variant(variant)) return fail(NTT_ERR_INVALID_ARG, "bad variant");
+    return run_config(d, cfg, false, true, 1, stream);
+}
+extern "C" ntt_status ntt_c_forward_negacyclic(uint64_t* d, const ntt_c_config* cfg, void* stream) {
+    return run_config(d, cfg, true, false, 1, stream);
+}
+extern "C" ntt_status ntt_c_inverse_negacyclic(uint64_t* d, const ntt_c_config* cfg, void* stream) {
+    return run_config(d, cfg, true, true, 1, stream);
+}
+extern "C" ntt_status ntt_c_forward_batch(uint64_t* d, uint64_t b, const ntt_c_config* cfg, int variant, void* stream) {
+    if (!valid_variant(variant)) return fail(NTT_ERR_INVALID_ARG, "bad variant");
+    return run_config(d, cfg, false, false, b, stream);
+}
+extern "C" ntt_status ntt_c_inverse_batch(uint64_t* d, uint64_t b, const ntt_c_config* cfg, int variant, void* stream) {
+    if (!valid_variant(variant)) return fail(NTT_ERR_INVALID_ARG, "bad variant");
+    return run_config(d, cfg, false, true, b, stream);
+}
+
+static ntt_status pointwise(uint64_t* r, const uint64_t* a, const uint64_t* b, uint64_t n, uint64_t q,
+                            hipStream_t stream) {
+    const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 8192);
+    if (q < (1ull << 31)) {
+        hipLaunchKernelGGL(pointwise_mul_kernel<uint64_t>, dim3(blocks), dim3(256), 0, stream, r, a, b, n, q,
+                           (uint64_t)(((u128)1 << 64) / q));
+    } else {
+        uint64_t inv = 1;  // q^-1 mod 2^64 by Newton iteration
+        for (int i = 0; i < 6; i++) inv *= 2 - q * inv;
+        const uint64_t r2 = (uint64_t)(((u128)1 << 64) % q * (((u128)1 << 64) % q) % q);
+        hipLaunchKernelGGL(pointwise_mul62_kernel, dim3(blocks), dim3(256), 0, stream, r, a, b, n, q, 0 - inv, r2);
+    }
+    HIP_TRY(hipGetLastError());
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_poly_multiply(uint64_t* r, const uint64_t* a, const uint64_t* b, const ntt_c_config* cfg,
+                                          void* stream) {
+    if (!r || !a || !b || !cfg) return fail(NTT_ERR_INVALID_ARG, "poly_multiply: null argument");
+    return pointwise(r, a, b, cfg->n, cfg->q, (hipStream_t)stream);
+}
+
+extern "C" ntt_status ntt_c_negacyclic_multiply(uint64_t* r, uint64_t* a, uint64_t* b, const ntt_c_config* cfg,
+                                                void* stream) {
+    if (!r || !a || !b || !cfg) return fail(NTT_ERR_INVALID_ARG, "negacyclic_multiply: null argument");
+    ntt_status st = ntt_c_forward_negacyclic(a, cfg, stream);
+    if (st == NTT_OK) st = ntt_c_forward_negacyclic(b, cfg, stream);
+    if (st == NTT_OK) st = pointwise(r, a, b, cfg->n, cfg->q, (hipStream_t)stream);
+    if (st == NTT_OK) st = ntt_c_inverse_negacyclic(r, cfg, stream);
+    return st;
+}
+
+extern "C" ntt_status ntt_c_bit_reverse(uint64_t* d, uint64_t n, void* stream) {
+    if (!d || n == 0 || (n & (n - 1)) || n > (1ull << 32)) return fail(NTT_ERR_INVALID_ARG, "bit_reverse: bad n");
+    const uint32_t log_n = (uint32_t)__builtin_ctzll(n);
+    const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(bitrev_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, d, n, log_n);
+    HIP_TRY(hipGetLastError());
+    return NTT_OK;
+}
+
+static ntt_status transpose(uint64_t* dst, const uint64_t* src, uint32_t n1, uint32_t n2, hipStream_t stream) {
+    dim3 grid((n2 + 31) / 32, (n1 + 31) / 32);
+    hipLaunchKernelGGL(transpose_u64_kernel, grid, dim3(32, 8), 0, stream, dst, src, n1, n2);
+    HIP_TRY(hipGetLastError());
+    return NTT_OK;
+}
+
+// Pre-transposed layout of the reference (src/ntt_optimized.cu:922-934):
+// pre[b*N1 + a] = canon[a*N2 + b].
+extern "C" ntt_status ntt_c_rowmaj_to_pretransposed(uint64_t* dst, const uint64_t* src, int n1, int n2, void* stream) {
+    if (!dst || !src || n1 <= 0 || n2 <= 0 || dst == src)
+        return fail(NTT_ERR_INVALID_ARG, "rowmaj_to_pretransposed: bad arguments (dst must differ from src)");
+    return transpose(dst, src, (uint32_t)n1, (uint32_t)n2, (hipStream_t)stream);
+}
+extern "C" ntt_status ntt_c_rowmaj_from_pretransposed(uint64_t* dst, const uint64_t* src, int n1, int n2,
+                                                      void* stream) {
+    if (!dst || !src || n1 <= 0 || n2 <= 0 || dst == src)
+        return fail(NTT_ERR_INVALID_ARG, "rowmaj_from_pretransposed: bad arguments (dst must differ from src)");
+    return transpose(dst, src, (uint32_t)n2, (uint32_t)n1, (hipStream_t)stream);
+}
+
+// ntt_forward_rowmaj: pre-transposed in, pre-transposed out.  Implemented as
+// layout change -> natural-order transform -> layout change through the
+// config's workspace.
+extern "C" ntt_status ntt_c_forward_rowmaj(uint64_t* d, const ntt_c_config* cfg, void* stream) {
+    if (!d || !cfg || !cfg->impl) return fail(NTT_ERR_INVALID_ARG, "forward_rowmaj: null argument");
+    ConfigImpl* I = (ConfigImpl*)cfg->impl;
+    const uint32_t l1 = (uint32_t)(cfg->log_n + 1) / 2, l2 = (uint32_t)cfg->log_n / 2;
+    const uint32_t n1 = 1u << l1, n2 = 1u << l2;
+    const Plan& P = *I->plans[0];
+    const uint64_t tbytes = cfg->n * sizeof(uint64_t);
+    const uint64_t sbytes = plan_scratch_elems(P) * vbytes(P.arith);
+    void* ws = nullptr;
+    ntt_status st = scratch_for(I, (hipStream_t)stream, tbytes + sbytes, &ws);
+    if (st != NTT_OK) return st;
+    uint64_t* tmp = (uint64_t*)ws;
+    st = transpose(tmp, d, n2, n1, (hipStream_t)stream);  // pre [N2][N1] -> canon
+    if (st == NTT_OK) st = run_plan(P, tmp, Store::U64, 1, (char*)ws + tbytes, (hipStream_t)stream);
+    if (st == NTT_OK) st = transpose(d, tmp, n1, n2, (hipStream_t)stream);  // canon -> pre
+    return st;
+}
+
+// ntt_host, reference src/ntt_basic.cu:412-463 — the public host transform.
+extern "C" ntt_status ntt_c_host(uint64_t* data, uint64_t n, const ntt_c_modular_params* prm, int inverse) {
+    if (!data || !prm || n == 0 || (n & (n - 1))) return fail(NTT_ERR_INVALID_ARG, "ntt_host: bad arguments");
+    const int log_n = __builtin_ctzll(n);
+    const uint64_t q = prm->q;
+    const uint64_t omega = inverse ? prm->omega_inv : prm->omega;
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t r = log_n ? (uint64_t)bitrev((uint32_t)i, (uint32_t)log_n) : 0;
+        if (i < r) std::swap(data[i], data[r]);
+    }
+    for (int s = 0; s < log_n; s++) {
+        const uint64_t half = 1ull << s, full = half << 1;
+        const uint64_t wb = powmod(omega, n / full, q);
+        for (uint64_t g = 0; g < n; g += full) {
+            uint64_t w = 1;
+            for (uint64_t p = 0; p < half; p++) {
+                const uint64_t u = data[g + p], v = mulmod(data[g + p + half], w, q);
+                data[g + p] = (u + v) % q;
+                data[g + p + half] = u >= v ? u - v : u + q - v;
+                w = mulmod(w, wb, q);
+            }
+        }
+    }
+    if (inverse)
+        for (uint64_t i = 0; i < n; i++) data[i] = mulmod(data[i], prm->n_inv, q);
+    return NTT_OK;
+}
+
+// ------------------------------------------------------------------ context + descriptor batches
+struct ntt_c_context {
+    std::mutex mu;
+    std::map<std::tuple<uint64_t, uint32_t, int, int, int>, std::unique_ptr<Plan>> plans;
+    uint64_t table_bytes = 0;
+    std::mutex smu;
+    std::map<hipStream_t, std::pair<void*, uint64_t>> scratch;  // per-call four-step workspace per stream
+    ~ntt_c_context() {
+        for (auto& kv : scratch)
+            if (kv.second.first) (void)hipFree(kv.second.first);
+    }
+};
+
+static Arith arith_for(uint64_t q, uint32_t flags) {
+    (void)flags;
+    return q < (1ull << 31) ? Arith::M32 : Arith::M64;
+}
+
+static ntt_status ctx_plan(ntt_c_context* ctx, const ntt_c_desc& d, const Plan** out) {
+    const bool nega = d.flags & NTT_FLAG_NEGACYCLIC, inv = d.flags & NTT_FLAG_INVERSE;
+    const Arith ar = arith_for(d.q, d.flags);
+    auto key = std::make_tuple(d.q, d.log_n, (int)nega, (int)inv, (int)ar);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->plans.find(key);
+    if (it == ctx->plans.end()) {
+        std::unique_ptr<Plan> P;
+        ntt_status st = build_plan(d.q, d.log_n, nega, inv, ar, P);
+        if (st != NTT_OK) return st;
+        ctx->table_bytes += P->bytes;
+        it = ctx->plans.emplace(key, std::move(P)).first;
+    }
+    *out = it->second.get();
+    return NTT_OK;
+}
+
+static ntt_status check_desc(const ntt_c_desc& d) {
+    if (!d.data) return fail(NTT_ERR_INVALID_ARG, "descriptor: null data");
+    if (d.flags & ~7u) return fail(NTT_ERR_INVALID_ARG, "descriptor: unknown flags");
+    if (!(d.flags & NTT_FLAG_U64) && d.q >= (1ull << 31))
+        return fail(NTT_ERR_INVALID_ARG, "descriptor: q >= 2^31 needs NTT_FLAG_U64 storage");
+    if (d.log_n < 1 || d.log_n > 26) return fail(NTT_ERR_UNSUPPORTED, "descriptor: log_n outside [1,26]");
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_context_create(ntt_c_context** out) {
+    if (!out) return fail(NTT_ERR_INVALID_ARG, "null output");
+    *out = new ntt_c_context();
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_context_destroy(ntt_c_context* ctx) {
+    if (ctx) {
+        (void)hipDeviceSynchronize();
+        delete ctx;
+    }
+    return NTT_OK;
+}
+extern "C" uint64_t ntt_c_context_table_bytes(const ntt_c_context* ctx) { return ctx ? ctx->table_bytes : 0; }
+
+struct ntt_c_batch {
+    // Launch groups: key (arith, in store, out store), a device job array.
+    struct Group {
+        Arith ar;
+        Store sin, sout;
+        uint32_t njobs = 0, nblocks = 0, lds = 0;
+        PassJob* djobs = nullptr;
+        PassJob inl{};
+    };
+    std::vector<Group> phases[2];  // phase 0: single-pass + pass 1; phase 1: pass 2
+    void* scratch = nullptr;
+    uint64_t scratch_bytes = 0;
+    uint64_t workgroups = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~ntt_c_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& ph : phases)
+            for (auto& g : ph)
+                if (g.djobs) (void)hipFree(g.djobs);
+        if (scratch) (void)hipFree(scratch);
+    }
+};
+
+extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count,
+                                          ntt_c_batch** out) {
+    if (!ctx || !out || (count && !descs)) return fail(NTT_ERR_INVALID_ARG, "batch_prepare: null argument");
+    *out = nullptr;
+    auto B = std::make_unique<ntt_c_batch>();
+    struct Item { PassJob job; Arith ar; Store sin, sout; int phase; };
+    std::vector<Item> items;
+    std::vector<const Plan*> plans(count);
+    uint64_t scratch_bytes = 0;
+    for (size_t i = 0; i < count; i++) {
+        ntt_status st = check_desc(descs[i]);
+        if (st != NTT_OK) return st;
+        st = ctx_plan(ctx, descs[i], &plans[i]);
+        if (st != NTT_OK) return st;
+        scratch_bytes += ((plan_scratch_elems(*plans[i]) * vbytes(plans[i]->arith)) + 255) & ~255ull;
+    }
+    if (scratch_bytes) {
+        hipError_t e = hipMalloc(&B->scratch, scratch_bytes);
+        if (e != hipSuccess) return fail(NTT_ERR_OOM, "batch_prepare: scratch allocation failed");
+        B->scratch_bytes = scratch_bytes;
+    }
+    uint64_t soff = 0;
+    for (size_t i = 0; i < count; i++) {
+        const Plan& P = *plans[i];
+        const Store st = (descs[i].flags & NTT_FLAG_U64) ? Store::U64 : Store::U32;
+        const Store mid = P.arith == Arith::M32 ? Store::U32 : Store::U64;
+        void* scr = nullptr;
+        if (P.npass == 2) {
+            scr = (char*)B->scratch + soff;
+            soff += ((plan_scratch_elems(P) * vbytes(P.arith)) + 255) & ~255ull;
+        }
+        PassJob jobs[2];
+        make_jobs(P, descs[i].data, st, 1, scr, jobs);
+        if (P.npass == 1) {
+            items.push_back({jobs[0], P.arith, st, st, 0});
+        } else {
+            items.push_back({jobs[0], P.arith, st, mid, 0});
+            items.push_back({jobs[1], P.arith, mid, st, 1});
+        }
+    }
+    // Group by (phase, arith, storage); within a group order by LDS size so
+    // that big tiles start first.
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+        return std::make_tuple(a.phase, (int)a.ar, (int)a.sin, (int)a.sout, -(int)(a.job.log_r + a.job.log_c)) <
+               std::make_tuple(b.phase, (int)b.ar, (int)b.sin, (int)b.sout, -(int)(b.job.log_r + b.job.log_c));
+    });
+    for (size_t i = 0; i < items.size();) {
+        size_t j = i;
+        ntt_c_batch::Group g;
+        g.ar = items[i].ar;
+        g.sin = items[i].sin;
+        g.sout = items[i].sout;
+        std::vector<PassJob> jobs;
+        uint32_t blocks = 0, lds = 0;
+        while (j < items.size() && items[j].phase == items[i].phase && items[j].ar == g.ar && items[j].sin == g.sin &&
+               items[j].sout == g.sout) {
+            PassJob J = items[j].job;
+            J.tile_begin = blocks;
+            blocks += job_tiles(J);
+            lds = std::max(lds, job_lds(J, g.ar));
+            jobs.push_back(J);
+            j++;
+        }
+        g.njobs = (uint32_t)jobs.size();
+        g.nblocks = blocks;
+        g.lds = lds;
+        hipError_t e = hipMalloc(&g.djobs, jobs.size() * sizeof(PassJob));
+        if (e != hipSuccess) return fail(NTT_ERR_OOM, "batch_prepare: job table allocation failed");
+        e = hipMemcpy(g.djobs, jobs.data(), jobs.size() * sizeof(PassJob), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(NTT_ERR_DEVICE, "batch_prepare: job table upload failed");
+        B->workgroups += blocks;
+        B->phases[items[i].phase].push_back(g);
+        i = j;
+    }
+    *out = B.release();
+    return NTT_OK;
+}
+
+static ntt_status batch_enqueue(ntt_c_batch* B, hipStream_t stream) {
+    for (auto& ph : B->phases)
+        for (auto& g : ph) {
+            ntt_status st = launch_pass(g.ar, g.sin, g.sout, g.djobs, g.njobs, g.inl, g.nblocks, g.lds, stream);
+            if (st != NTT_OK) return st;
+        }
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_batch_launch(ntt_c_batch* B, void* stream) {
+    if (!B) return fail(NTT_ERR_INVALID_ARG, "batch_launch: null batch");
+    return batch_enqueue(B, (hipStream_t)stream);
+}
+
+extern "C" ntt_status ntt_c_batch_launch_graph(ntt_c_batch* B, void* stream) {
+    if (!B) return fail(NTT_ERR_INVALID_ARG, "batch_launch_graph: null batch");
+    if (!B->exec) {
+        hipStream_t cap = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            (void)hipStreamDestroy(cap);
+            return fail(NTT_ERR_DEVICE, "hipStreamBeginCapture failed");
+        }
+        ntt_status st = batch_enqueue(B, cap);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(cap, &graph);
+        (void)hipStreamDestroy(cap);
+        if (st != NTT_OK) return st;
+        if (e != hipSuccess) return fail(NTT_ERR_DEVICE, "hipStreamEndCapture failed");
+        B->graph = graph;
+        HIP_TRY(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(B->exec, (hipStream_t)stream));
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_batch_destroy(ntt_c_batch* B) {
+    if (B) {
+        (void)hipDeviceSynchronize();
+        delete B;
+    }
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_batch_info(const ntt_c_batch* B, uint32_t* launches, uint64_t* ws, uint64_t* wgs) {
+    if (!B) return fail(NTT_ERR_INVALID_ARG, "batch_info: null batch");
+    if (launches) *launches = (uint32_t)(B->phases[0].size() + B->phases[1].size());
+    if (ws) *ws = B->scratch_bytes;
+    if (wgs) *wgs = B->workgroups;
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* d, void* stream) {
+    if (!ctx || !d) return fail(NTT_ERR_INVALID_ARG, "transform: null argument");
+    ntt_status st = check_desc(*d);
+    if (st != NTT_OK) return st;
+    const Plan* P = nullptr;
+    st = ctx_plan(ctx, *d, &P);
+    if (st != NTT_OK) return st;
+    // Per-call path: the four-step workspace comes from a context-owned
+    // per-stream buffer (no allocation once it is large enough).
+    void* scr = nullptr;
+    const uint64_t need = plan_scratch_elems(*P) * vbytes(P->arith);
+    if (need) {
+        std::lock_guard<std::mutex> lk(ctx->smu);
+        auto& ent = ctx->scratch[(hipStream_t)stream];
+        if (ent.second < need) {
+            if (ent.first) {
+                HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+                HIP_TRY(hipFree(ent.first));
+            }
+            ent.first = nullptr;
+            ent.second = 0;
+            if (hipMalloc(&ent.first, need) != hipSuccess) return fail(NTT_ERR_OOM, "transform: scratch allocation failed");
+            ent.second = need;
+        }
+        scr = ent.first;
+    }
+    return run_plan(*P, d->data, (d->flags & NTT_FLAG_U64) ? Store::U64 : Store::U32, 1, scr, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ device helpers
+extern "C" ntt_status ntt_c_device_count(int* c) {
+    if (!c) return fail(NTT_ERR_INVALID_ARG, "null");
+    *c = 0;
+    HIP_TRY(hipGetDeviceCount(c));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_malloc(void** p, uint64_t bytes) {
+    if (!p) return fail(NTT_ERR_INVALID_ARG, "null");
+    if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) return fail(NTT_ERR_OOM, "hipMalloc failed");
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_free(void* p) {
+    if (p) HIP_TRY(hipFree(p));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_memcpy_h2d(void* dst, const void* src, uint64_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_memset(void* dst, int value, uint64_t bytes) {
+    HIP_TRY(hipMemset(dst, value, bytes));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_stream_create(void** s) {
+    if (!s) return fail(NTT_ERR_INVALID_ARG, "null");
+    HIP_TRY(hipStreamCreateWithFlags((hipStream_t*)s, hipStreamNonBlocking));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_stream_destroy(void* s) {
+    if (s) HIP_TRY(hipStreamDestroy((hipStream_t)s));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_stream_sync(void* s) {
+    HIP_TRY(hipStreamSynchronize((hipStream_t)s));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_device_sync(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_event_create(void** e) {
+    if (!e) return fail(NTT_ERR_INVALID_ARG, "null");
+    HIP_TRY(hipEventCreate((hipEvent_t*)e));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_event_destroy(void* e) {
+    if (e) HIP_TRY(hipEventDestroy((hipEvent_t)e));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_event_record(void* e, void* s) {
+    HIP_TRY(hipEventRecord((hipEvent_t)e, (hipStream_t)s));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_event_elapsed_ms(float* ms, void* a, void* b) {
+    if (!ms) return fail(NTT_ERR_INVALID_ARG, "null");
+    HIP_TRY(hipEventSynchronize((hipEvent_t)b));
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return NTT_OK;
+}

@@ -1,0 +1,340 @@
+"""ctypes binding of libntt_mi355x.so (include/ntt_c.h) for Python hosts.
+
+The library is the product; this module only marshals pointers and sizes.
+Importing it never falls back to anything: if the shared library is missing
+it raises, so no caller can run without the HIP kernels.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libntt_mi355x.so")
+HEADER = os.path.join(REPO, "include", "ntt_c.h")
+
+NTT_OK = 0
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "NO_ROOT", 3: "OOM", 4: "DEVICE", 5: "UNSUPPORTED", 6: "BUSY"}
+VARIANT = {"basic": 0, "four_step": 1, "tlmop": 2, "optimized": 3}
+FLAG_INVERSE, FLAG_NEGACYCLIC, FLAG_U64 = 1, 2, 4
+
+BABYBEAR = 2013265921
+
+
+class NttError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class ModularParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("q", "q_twice", "barrett_k", "barrett_m", "omega", "omega_inv", "n_inv")]
+
+
+class ConfigStruct(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("log_n", ctypes.c_uint64), ("q", ctypes.c_uint64),
+                ("mu", ctypes.c_uint64), ("d_twiddles", ctypes.c_void_p),
+                ("d_twiddles_inv", ctypes.c_void_p), ("params", ModularParams),
+                ("impl", ctypes.c_void_p)]
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("q", ctypes.c_uint64), ("log_n", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def header_symbols():
+    """Names of every function declared in include/ntt_c.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ntt_c_\w+)\s*\(", text)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `make -C {HERE}` (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
+    P = ctypes.POINTER
+    sigs = {
+        "ntt_c_init": [u64, u64, P(P(ConfigStruct))],
+        "ntt_c_cleanup": [P(ConfigStruct)],
+        "ntt_c_forward": [vp, P(ConfigStruct), i32, vp],
+        "ntt_c_inverse": [vp, P(ConfigStruct), i32, vp],
+        "ntt_c_forward_negacyclic": [vp, P(ConfigStruct), vp],
+        "ntt_c_inverse_negacyclic": [vp, P(ConfigStruct), vp],
+        "ntt_c_forward_batch": [vp, u64, P(ConfigStruct), i32, vp],
+        "ntt_c_inverse_batch": [vp, u64, P(ConfigStruct), i32, vp],
+        "ntt_c_poly_multiply": [vp, vp, vp, P(ConfigStruct), vp],
+        "ntt_c_negacyclic_multiply": [vp, vp, vp, P(ConfigStruct), vp],
+        "ntt_c_bit_reverse": [vp, u64, vp],
+        "ntt_c_forward_rowmaj": [vp, P(ConfigStruct), vp],
+        "ntt_c_rowmaj_to_pretransposed": [vp, vp, i32, i32, vp],
+        "ntt_c_rowmaj_from_pretransposed": [vp, vp, i32, i32, vp],
+        "ntt_c_host": [P(u64), u64, P(ModularParams), i32],
+        "ntt_c_create_params": [u64, u64, P(ModularParams)],
+        "ntt_c_context_create": [P(vp)],
+        "ntt_c_context_destroy": [vp],
+        "ntt_c_transform": [vp, P(Desc), vp],
+        "ntt_c_batch_prepare": [vp, P(Desc), ctypes.c_size_t, P(vp)],
+        "ntt_c_batch_launch": [vp, vp],
+        "ntt_c_batch_launch_graph": [vp, vp],
+        "ntt_c_batch_destroy": [vp],
+        "ntt_c_batch_info": [vp, P(u32), P(u64), P(u64)],
+        "ntt_c_device_count": [P(i32)],
+        "ntt_c_malloc": [P(vp), u64],
+        "ntt_c_free": [vp],
+        "ntt_c_memcpy_h2d": [vp, vp, u64],
+        "ntt_c_memcpy_d2h": [vp, vp, u64],
+        "ntt_c_memset": [vp, i32, u64],
+        "ntt_c_stream_create": [P(vp)],
+        "ntt_c_stream_destroy": [vp],
+        "ntt_c_stream_sync": [vp],
+        "ntt_c_device_sync": [],
+        "ntt_c_event_create": [P(vp)],
+        "ntt_c_event_destroy": [vp],
+        "ntt_c_event_record": [vp, vp],
+        "ntt_c_event_elapsed_ms": [P(ctypes.c_float), vp, vp],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.ntt_c_last_error.restype = ctypes.c_char_p
+    L.ntt_c_last_error.argtypes = []
+    L.ntt_c_find_primitive_root.restype = u64
+    L.ntt_c_find_primitive_root.argtypes = [u64, u64]
+    L.ntt_c_context_table_bytes.restype = u64
+    L.ntt_c_context_table_bytes.argtypes = [vp]
+    for extra in _EXTRA_SIGS:
+        extra(L)
+    _lib = L
+    return L
+
+
+# Later modules of the binding register more prototypes here.
+_EXTRA_SIGS = []
+
+
+def check(status):
+    if status != NTT_OK:
+        raise NttError(status, lib().ntt_c_last_error().decode(errors="replace"))
+
+
+# ---------------------------------------------------------------- device memory
+class DeviceBuffer:
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().ntt_c_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a, offset=0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        check(lib().ntt_c_memcpy_h2d(self.ptr + offset, a.ctypes.data, a.nbytes))
+
+    def download(self, dtype, count, offset=0):
+        out = np.empty(count, dtype=dtype)
+        assert offset + out.nbytes <= self.nbytes
+        check(lib().ntt_c_memcpy_d2h(out.ctypes.data, self.ptr + offset, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib().ntt_c_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.ntt_c_free(self.ptr)
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().ntt_c_stream_create(ctypes.byref(p)))
+        self.ptr = p.value
+
+    def sync(self):
+        check(lib().ntt_c_stream_sync(self.ptr))
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().ntt_c_stream_destroy(self.ptr))
+            self.ptr = None
+
+
+class Event:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().ntt_c_event_create(ctypes.byref(p)))
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        check(lib().ntt_c_event_record(self.ptr, stream.ptr if stream else None))
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        check(lib().ntt_c_event_elapsed_ms(ctypes.byref(ms), self.ptr, end.ptr))
+        return ms.value
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().ntt_c_event_destroy(self.ptr))
+            self.ptr = None
+
+
+def device_count():
+    c = ctypes.c_int()
+    check(lib().ntt_c_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def device_sync():
+    check(lib().ntt_c_device_sync())
+
+
+def _sp(stream):
+    return stream.ptr if stream is not None else None
+
+
+# ---------------------------------------------------------------- include/ntt.cuh surface
+class Config:
+    """NTTConfig from ntt_init (reference src/ntt_basic.cu:217-249)."""
+
+    def __init__(self, n, q=BABYBEAR):
+        p = ctypes.POINTER(ConfigStruct)()
+        check(lib().ntt_c_init(n, q, ctypes.byref(p)))
+        self.ptr = p
+        self.n, self.q = n, q
+
+    @property
+    def params(self):
+        return self.ptr.contents.params
+
+    @property
+    def mu(self):
+        return self.ptr.contents.mu
+
+    def forward(self, buf, variant="optimized", stream=None):
+        check(lib().ntt_c_forward(buf.ptr, self.ptr, VARIANT[variant], _sp(stream)))
+
+    def inverse(self, buf, variant="optimized", stream=None):
+        check(lib().ntt_c_inverse(buf.ptr, self.ptr, VARIANT[variant], _sp(stream)))
+
+    def forward_negacyclic(self, buf, stream=None):
+        check(lib().ntt_c_forward_negacyclic(buf.ptr, self.ptr, _sp(stream)))
+
+    def inverse_negacyclic(self, buf, stream=None):
+        check(lib().ntt_c_inverse_negacyclic(buf.ptr, self.ptr, _sp(stream)))
+
+    def forward_batch(self, buf, batch, variant="optimized", stream=None):
+        check(lib().ntt_c_forward_batch(buf.ptr, batch, self.ptr, VARIANT[variant], _sp(stream)))
+
+    def inverse_batch(self, buf, batch, variant="optimized", stream=None):
+        check(lib().ntt_c_inverse_batch(buf.ptr, batch, self.ptr, VARIANT[variant], _sp(stream)))
+
+    def poly_multiply(self, out, a, b, stream=None):
+        check(lib().ntt_c_poly_multiply(out.ptr, a.ptr, b.ptr, self.ptr, _sp(stream)))
+
+    def negacyclic_multiply(self, out, a, b, stream=None):
+        check(lib().ntt_c_negacyclic_multiply(out.ptr, a.ptr, b.ptr, self.ptr, _sp(stream)))
+
+    def forward_rowmaj(self, buf, stream=None):
+        check(lib().ntt_c_forward_rowmaj(buf.ptr, self.ptr, _sp(stream)))
+
+    def cleanup(self):
+        if self.ptr:
+            check(lib().ntt_c_cleanup(self.ptr))
+            self.ptr = None
+
+
+def create_params(q, n):
+    p = ModularParams()
+    check(lib().ntt_c_create_params(q, n, ctypes.byref(p)))
+    return p
+
+
+def ntt_host(data, params, inverse=False):
+    """The library's host transform (reference ntt_host), on a copy."""
+    a = np.ascontiguousarray(data, dtype=np.uint64).copy()
+    check(lib().ntt_c_host(a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), a.size, ctypes.byref(params),
+                           int(inverse)))
+    return a
+
+
+def bit_reverse(buf, n, stream=None):
+    check(lib().ntt_c_bit_reverse(buf.ptr, n, _sp(stream)))
+
+
+def rowmaj_to_pretransposed(dst, src, n1, n2, stream=None):
+    check(lib().ntt_c_rowmaj_to_pretransposed(dst.ptr, src.ptr, n1, n2, _sp(stream)))
+
+
+def rowmaj_from_pretransposed(dst, src, n1, n2, stream=None):
+    check(lib().ntt_c_rowmaj_from_pretransposed(dst.ptr, src.ptr, n1, n2, _sp(stream)))
+
+
+# ---------------------------------------------------------------- descriptor API
+class Context:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().ntt_c_context_create(ctypes.byref(p)))
+        self.ptr = p.value
+
+    def table_bytes(self):
+        return int(lib().ntt_c_context_table_bytes(self.ptr))
+
+    def transform(self, data_ptr, q, log_n, flags, stream=None):
+        d = Desc(data_ptr, q, log_n, flags)
+        check(lib().ntt_c_transform(self.ptr, ctypes.byref(d), _sp(stream)))
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().ntt_c_context_destroy(self.ptr))
+            self.ptr = None
+
+
+class Batch:
+    def __init__(self, ctx, descs):
+        """descs: list of (data_ptr, q, log_n, flags)."""
+        arr = (Desc * len(descs))(*[Desc(*d) for d in descs])
+        p = ctypes.c_void_p()
+        check(lib().ntt_c_batch_prepare(ctx.ptr, arr, len(descs), ctypes.byref(p)))
+        self.ptr = p.value
+
+    def launch(self, stream=None):
+        check(lib().ntt_c_batch_launch(self.ptr, _sp(stream)))
+
+    def launch_graph(self, stream=None):
+        check(lib().ntt_c_batch_launch_graph(self.ptr, _sp(stream)))
+
+    def info(self):
+        l, w, g = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().ntt_c_batch_info(self.ptr, ctypes.byref(l), ctypes.byref(w), ctypes.byref(g)))
+        return {"launches": l.value, "workspace_bytes": w.value, "workgroups": g.value}
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().ntt_c_batch_destroy(self.ptr))
+            self.ptr = None

@@ -1,0 +1,91 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol that
+include/ntt_c.h declares, and its host-only functions (params, ntt_host) and
+argument validation behave like the reference.  No kernel is launched."""
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+BB = 2013265921
+
+
+def test_library_exports_every_declared_symbol(nt):
+    syms = nt.header_symbols()
+    assert len(syms) >= 40
+    L = nt.lib()
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_exports_are_c_linkage():
+    out = subprocess.check_output(["nm", "-D", "--defined-only", __import__("ntt_amd").LIB_PATH], text=True)
+    names = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    import ntt_amd
+    for s in ntt_amd.header_symbols():
+        assert s in names, s  # unmangled
+
+
+def test_create_params_matches_oracle(nt, orc):
+    for q, n in [(BB, 4096), (BB, 1 << 20), (1056440321, 1 << 18), (4611686018326724609, 1 << 22)]:
+        a = nt.create_params(q, n)
+        b = orc.params(q, n)
+        for f in ("q", "q_twice", "barrett_k", "barrett_m", "omega", "omega_inv", "n_inv"):
+            assert getattr(a, f) == getattr(b, f), (q, n, f)
+
+
+@pytest.mark.parametrize("q", [BB, 4611686018326724609])
+@pytest.mark.parametrize("log_n", [1, 6, 12])
+def test_library_ntt_host_matches_oracle(nt, orc, q, log_n):
+    n = 1 << log_n
+    x = orc.random_poly(n, q, 3)
+    p = nt.create_params(q, n)
+    assert np.array_equal(nt.ntt_host(x, p), orc.ntt(x, q))
+    assert np.array_equal(nt.ntt_host(x, p, inverse=True), orc.ntt(x, q, inverse=True))
+
+
+def test_find_primitive_root(nt, orc):
+    L = nt.lib()
+    for q, n in [(BB, 1 << 12), (BB, 1 << 24), (2147352577, 1 << 16)]:
+        assert L.ntt_c_find_primitive_root(q, n) == orc.omega(q, n)
+
+
+def test_init_rejects_bad_arguments_before_touching_the_device(nt):
+    L = nt.lib()
+    cfg = ctypes.POINTER(nt.ConfigStruct)()
+    assert L.ntt_c_init(3, BB, ctypes.byref(cfg)) == 1            # n not a power of two
+    assert L.ntt_c_init(0, BB, ctypes.byref(cfg)) == 1
+    assert L.ntt_c_init(1024, BB + 2, ctypes.byref(cfg)) == 2     # q not prime
+    assert L.ntt_c_init(1 << 28, BB, ctypes.byref(cfg)) == 2      # 2^28 does not divide q-1
+    assert L.ntt_c_init(1024, 1 << 62, ctypes.byref(cfg)) == 5    # q too large
+    assert b"prime" in L.ntt_c_last_error() or b"q" in L.ntt_c_last_error()
+    assert not cfg
+
+
+def test_descriptor_validation_is_host_side(nt):
+    ctx = nt.Context()
+    with pytest.raises(nt.NttError) as e:
+        ctx.transform(0, BB, 10, 0)                                  # null data
+    assert e.value.status == 1
+    with pytest.raises(nt.NttError) as e:
+        ctx.transform(0x1000, 4611686018326724609, 10, 0)            # 62-bit q on u32 storage
+    assert e.value.status == 1
+    with pytest.raises(nt.NttError) as e:
+        ctx.transform(0x1000, BB, 10, 64)                            # unknown flag
+    assert e.value.status == 1
+    with pytest.raises(nt.NttError) as e:
+        ctx.transform(0x1000, BB, 27, 0)                             # n too large
+    assert e.value.status == 5
+    with pytest.raises(nt.NttError) as e:
+        ctx.transform(0x1000, 1073479681, 21, 2)                     # 2n does not divide q-1
+    assert e.value.status == 2
+    ctx.destroy()
+
+
+def test_null_arguments_rejected(nt):
+    L = nt.lib()
+    assert L.ntt_c_forward(None, None, 3, None) == 1
+    assert L.ntt_c_forward(ctypes.c_void_p(16), None, 9, None) == 1   # bad variant
+    assert L.ntt_c_bit_reverse(None, 8, None) == 1
+    assert L.ntt_c_bit_reverse(ctypes.c_void_p(16), 6, None) == 1
+    assert L.ntt_c_rowmaj_to_pretransposed(ctypes.c_void_p(16), ctypes.c_void_p(16), 64, 64, None) == 1
