@@ -148,6 +148,14 @@ uint64_t ntt_c_context_table_bytes(const ntt_c_context* ctx);
 /* One transform, one polynomial, launched on its own (the per-polynomial path). */
 ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* desc, void* stream);
 
+/* One transform with an explicit radix split n = 2^log_r[0] * ... * 2^log_r[npass-1]
+ * (npass <= 4) and a tile of 2^log_e elements per workgroup.  No reference
+ * counterpart (the reference fixes its split at radix-64 rounds,
+ * src/ntt_optimized.cu:529-554); used by the autotuner and by tests that
+ * cover every plan shape. */
+ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc* desc, uint32_t npass, const uint32_t* log_r,
+                                 uint32_t log_e, void* stream);
+
 /* Prepare a batch of descriptors (mixed n, q, direction, transform type):
  * builds tables for every (q, n) in the batch, the device job lists and the
  * workspace.  The descriptors' data pointers are bound at prepare time. */

@@ -100,32 +100,79 @@ __global__ void powers_kernel(uint64_t* out, uint64_t n, const TwP<typename M::V
 }  // namespace nttk
 
 // ------------------------------------------------------------------ device limits
-static int g_max_lds = -1;
-
-static int device_max_lds() {
-    if (g_max_lds < 0) {
-        int dev = 0, v = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
-            g_max_lds = v;
-        else
-            g_max_lds = 65536;
-    }
-    return g_max_lds;
-}
-
-// Tile budget (elements of the arithmetic type) for one workgroup's LDS tile.
-static const uint32_t kLdsBytes = 65536;
+// LDS budget of one workgroup's tile (the CT table may add up to 16 KB).
+static const uint32_t kTileBytes = 65536;
+static const uint32_t kLdsMax = 81920;
+static const int kThreads = 512;
 
 // ------------------------------------------------------------------ plans
 enum class Arith { M32, M64 };
+enum class Store { U32, U64 };
+
+static uint32_t vbytes(Arith a) { return a == Arith::M32 ? 4u : 8u; }
+static uint32_t sbytes(Store s) { return s == Store::U32 ? 4u : 8u; }
+
+// A radix split: P passes of 2^log_r[i] points, tile 2^log_e elements.
+struct Split {
+    uint32_t npass = 1;
+    uint32_t log_r[4] = {0, 0, 0, 0};
+    uint32_t log_e = 14;
+    bool operator<(const Split& o) const {
+        return std::make_tuple(npass, log_r[0], log_r[1], log_r[2], log_r[3], log_e) <
+               std::make_tuple(o.npass, o.log_r[0], o.log_r[1], o.log_r[2], o.log_r[3], o.log_e);
+    }
+};
+
+static uint32_t max_log_e(Arith a) { return a == Arith::M32 ? 14u : 13u; }
+
+// Autotuned choices, keyed by (arith, log_n, log2 of the elements in flight);
+// filled by ntt_c_autotune, consulted before the heuristic.
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, uint32_t, uint32_t>, Split> g_tuned;
+
+static uint32_t ilog2_floor(uint64_t v) { return v ? 63u - (uint32_t)__builtin_clzll(v) : 0u; }
+
+// Heuristic split: the fewest passes whose radices leave >= 16 columns per tile
+// (64 B row segments), with the tile as large as keeps >= 256 workgroups busy
+// for the elements in flight in the launch.
+static Split heuristic_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight) {
+    Split sp;
+    const uint32_t emax = max_log_e(ar);
+    const uint32_t lf = ilog2_floor(std::max<uint64_t>(elems_in_flight, 1ull << log_n));
+    // largest tile that still leaves >= 256 workgroups (never below 2^11 elements)
+    const uint32_t le_par = std::min(emax, std::max(lf > 8 ? lf - 8 : 0u, 11u));
+    if (log_n <= emax - 2) {
+        sp.npass = 1;
+        sp.log_r[0] = log_n;
+        sp.log_e = std::max(log_n, le_par);
+        return sp;
+    }
+    for (uint32_t P = 2; P <= 4; P++) {
+        const uint32_t lr_max = (log_n + P - 1) / P;
+        if (lr_max + 4 > le_par && P < 4) continue;  // keep >= 16 columns per tile
+        sp.npass = P;
+        for (uint32_t i = 0; i < P; i++) sp.log_r[i] = log_n / P + (i < log_n % P ? 1u : 0u);
+        sp.log_e = std::min(emax, std::max(le_par, lr_max + 4));
+        return sp;
+    }
+    return sp;
+}
+
+static Split choose_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight) {
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(std::make_tuple((int)ar, log_n, ilog2_floor(std::max<uint64_t>(elems_in_flight, 1))));
+        if (it != g_tuned.end()) return it->second;
+    }
+    return heuristic_split(log_n, ar, elems_in_flight);
+}
 
 struct PassSpec {
-    uint32_t log_r = 0, log_c = 0;
-    int z_off = -1;          // byte offset of the CT table in the plan's block
+    uint32_t log_r = 0, log_c = 0, log_l = 0;  // rows, tile columns, product of earlier radices
+    int z_off = -1;
     uint32_t post = POST_NONE;
     int lo_off = -1, hi_off = -1;
-    uint32_t e_a = 0, e_b = 0, e_d = 0, e_mask = 0, lo_bits = 0;
+    uint32_t e_a = 0, e_b = 0, e_d = 0, e_shift = 0, e_mask = 0, lo_bits = 0;
     uint64_t scale_w = 0, scale_wp = 0;
 };
 
@@ -134,24 +181,15 @@ struct Plan {
     uint32_t log_n = 0;
     bool nega = false, inv = false;
     Arith arith = Arith::M32;
+    Split split;
     int npass = 1;
-    PassSpec pass[2];
-    uint32_t log_n1 = 0, log_n2 = 0;  // two-pass split
-    void* block = nullptr;            // device tables
+    PassSpec pass[4];
+    void* block = nullptr;  // device tables
     uint64_t bytes = 0;
     ~Plan() {
         if (block) (void)hipFree(block);
     }
 };
-
-static uint32_t vbytes(Arith a) { return a == Arith::M32 ? 4u : 8u; }
-
-static uint32_t tile_log_elems(Arith a) {
-    uint32_t e = kLdsBytes / vbytes(a);
-    uint32_t l = 0;
-    while ((2u << l) <= e) l++;
-    return l;  // 14 for 32-bit values, 13 for 64-bit
-}
 
 // Validates (q, n) the way the transforms need: prime q, 2^log_n | q-1
 // (2^(log_n+1) for the negacyclic root).
@@ -186,26 +224,44 @@ struct TableBuilder {
         }
         return (int)off;
     }
+    // Two-level table of base^e, e < 2^mbits: low 2^lob entries and high part.
+    void pow_tables(PassSpec& s, uint64_t base, uint32_t mbits, uint64_t mult) {
+        const uint32_t lob = (mbits + 1) / 2;
+        s.e_mask = (uint32_t)((1ull << mbits) - 1);
+        s.lo_bits = lob;
+        s.lo_off = add(powers(base, 1ull << lob, q));
+        s.hi_off = add(powers(powmod(base, 1ull << lob, q), 1ull << (mbits - lob), q, mult));
+    }
 };
 
-// Builds the tables of one transform.  The formulas are the decompositions
-// checked against ntt_host (see the header of ntt_kernels.hpp):
-//   single pass        : CT table over N (root omega, factor psi if negacyclic);
-//                        inverse cyclic: scale n^-1; inverse negacyclic: row table psi^-k n^-1.
-//   two pass N = N1*N2 : pass 1 CT over N1 (root omega^N2, factor psi^N2 if forward
-//                        negacyclic) then four-step twiddle; pass 2 CT over N2 (root
-//                        omega^N1), then psi^(-N1 k) for the inverse negacyclic.
-static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Arith arith, std::unique_ptr<Plan>& out) {
+// Builds the tables of one transform for a radix split N = R_0 * ... * R_{P-1}.
+// Pass i runs R_i-point transforms; L_i = R_0*...*R_{i-1} and M_i = N / L_i.
+//   pass 0     : CT table over R_0 with root w^(N/R_0) (times psi^(N/R_0) for the
+//                forward negacyclic), then the four-step twiddle
+//                cyclic w^(k c) mod N | forward nega psi^(c(2k+1)) | inverse nega psi^(k(2c+1)),
+//                with n^-1 folded in for inverses; transposed store.
+//   pass 0<i<P-1: cyclic R_i-point, twiddle w_{M_i}^(k u), u = c >> log L_i.
+//   pass P-1   : cyclic R_{P-1}-point, in-place store; the inverse negacyclic
+//                ends with psi_{2 M_1}^(-(L_{P-1}/R_0) k - (c >> log R_0)).
+// With P = 1 the inverse scales by n^-1 (cyclic) or psi^-k n^-1 (negacyclic).
+// Checked against ntt_host in emulation of these exact index formulas.
+static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Arith arith, const Split& sp,
+                             std::unique_ptr<Plan>& out) {
     ntt_status st = check_modulus(q, log_n, nega);
     if (st != NTT_OK) return st;
     if (arith == Arith::M32 && q >= (1ull << 31)) return fail(NTT_ERR_UNSUPPORTED, "32-bit arithmetic needs q < 2^31");
     if (log_n < 1 || log_n > 26) return fail(NTT_ERR_UNSUPPORTED, "log2 n must be in [1, 26]");
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < sp.npass; i++) sum += sp.log_r[i];
+    if (sp.npass < 1 || sp.npass > 4 || sum != log_n) return fail(NTT_ERR_INVALID_ARG, "radix split does not cover n");
     auto P = std::make_unique<Plan>();
     P->q = q;
     P->log_n = log_n;
     P->nega = nega;
     P->inv = inv;
     P->arith = arith;
+    P->split = sp;
+    P->npass = (int)sp.npass;
     const uint64_t N = 1ull << log_n;
     const uint64_t g = generator(q);
     const uint64_t omega = powmod(g, (q - 1) / N, q);
@@ -213,53 +269,46 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
     const uint64_t w = inv ? invmod(omega, q) : omega;
     const uint64_t p = inv ? invmod(psi, q) : psi;
     const uint64_t ninv = invmod(N % q, q);
+    const int sbits = arith == Arith::M32 ? 32 : 64;
     TableBuilder tb{arith, q, {}};
-    const uint32_t tl = tile_log_elems(arith);
-
-    if (log_n <= tl) {
-        P->npass = 1;
-        PassSpec& s = P->pass[0];
-        s.log_r = log_n;
-        s.log_c = tl - log_n;  // several polynomials per tile when batched
-        s.z_off = tb.add(ct_table(log_n, w, (nega && !inv) ? p : 1, q));
-        if (inv && !nega) {
+    uint32_t log_l = 0;
+    for (int i = 0; i < P->npass; i++) {
+        PassSpec& s = P->pass[i];
+        const uint32_t lr = sp.log_r[i];
+        s.log_r = lr;
+        s.log_l = log_l;
+        s.log_c = std::min(sp.log_e > lr ? sp.log_e - lr : 0u, log_n - lr);
+        const uint64_t fac = (i == 0 && nega && !inv) ? powmod(p, N >> lr, q) : 1;
+        s.z_off = tb.add(ct_table(lr, powmod(w, N >> lr, q), fac, q));
+        if (i < P->npass - 1) {
+            s.post = POST_FOURSTEP;
+            if (i == 0) {
+                if (!nega) { s.e_a = 1; tb.pow_tables(s, w, log_n, inv ? ninv : 1); }
+                else if (!inv) { s.e_a = 2; s.e_d = 1; tb.pow_tables(s, p, log_n + 1, 1); }
+                else { s.e_a = 2; s.e_b = 1; tb.pow_tables(s, p, log_n + 1, ninv); }
+            } else {
+                s.e_a = 1;
+                s.e_shift = log_l;
+                tb.pow_tables(s, powmod(w, 1ull << log_l, q), log_n - log_l, 1);
+            }
+        } else if (inv && nega) {
+            s.post = POST_FOURSTEP;
+            if (P->npass == 1) {
+                s.e_b = 1;
+                tb.pow_tables(s, p, log_n + 1, ninv);
+            } else {
+                const uint32_t lr0 = sp.log_r[0];
+                s.e_b = 1u << (log_l - lr0);
+                s.e_d = 1;
+                s.e_shift = lr0;
+                tb.pow_tables(s, powmod(p, 1ull << lr0, q), log_n - lr0 + 1, 1);
+            }
+        } else if (inv && P->npass == 1) {
             s.post = POST_SCALE;
             s.scale_w = ninv;
-            s.scale_wp = shoup(ninv, q, arith == Arith::M32 ? 32 : 64);
-        } else if (inv && nega) {
-            s.post = POST_ROW;
-            s.lo_off = tb.add(powers(p, N, q, ninv));
+            s.scale_wp = shoup(ninv, q, sbits);
         }
-    } else {
-        P->npass = 2;
-        const uint32_t l1 = (log_n + 1) / 2, l2 = log_n / 2;
-        P->log_n1 = l1;
-        P->log_n2 = l2;
-        const uint64_t N1 = 1ull << l1, N2 = 1ull << l2;
-        PassSpec& a = P->pass[0];
-        a.log_r = l1;
-        a.log_c = std::min(tl - l1, l2);
-        a.z_off = tb.add(ct_table(l1, powmod(w, N2, q), (nega && !inv) ? powmod(p, N2, q) : 1, q));
-        a.post = POST_FOURSTEP;
-        // exponent modulus M = N (cyclic) or 2N (negacyclic), split in two tables
-        const uint32_t mbits = log_n + (nega ? 1 : 0);
-        const uint32_t lob = (mbits + 1) / 2;
-        const uint64_t base = nega ? p : w;
-        a.e_mask = (uint32_t)((1ull << mbits) - 1);
-        a.lo_bits = lob;
-        if (!nega) { a.e_a = 1; a.e_b = 0; a.e_d = 0; }        // omega^(k c)
-        else if (!inv) { a.e_a = 2; a.e_b = 0; a.e_d = 1; }    // psi^(c (2k+1))
-        else { a.e_a = 2; a.e_b = 1; a.e_d = 0; }              // psi^-(k (2c+1))
-        a.lo_off = tb.add(powers(base, 1ull << lob, q));
-        a.hi_off = tb.add(powers(powmod(base, 1ull << lob, q), 1ull << (mbits - lob), q, inv ? ninv : 1));
-        PassSpec& b = P->pass[1];
-        b.log_r = l2;
-        b.log_c = std::min(tl - l2, l1);
-        b.z_off = tb.add(ct_table(l2, powmod(w, N1, q), 1, q));
-        if (nega && inv) {
-            b.post = POST_ROW;
-            b.lo_off = tb.add(powers(powmod(p, N1, q), N2, q));
-        }
+        log_l += lr;
     }
     P->bytes = tb.host.size();
     hipError_t e = hipMalloc(&P->block, P->bytes);
@@ -271,64 +320,95 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
 }
 
 // Scratch elements (arithmetic width) one polynomial needs for this plan.
-static uint64_t plan_scratch_elems(const Plan& P) { return P.npass == 2 ? (1ull << P.log_n) : 0; }
+static uint64_t plan_scratch_elems(const Plan& P) { return P.npass >= 2 ? (1ull << P.log_n) : 0; }
 
-enum class Store { U32, U64 };
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// Load/store order, vector width and LDS staging flags of a job.
+static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout) {
+    const uint32_t R = 1u << J.log_r, C = 1u << J.log_c;
+    uint32_t f = 0;
+    const bool lcf = J.s_c == 1;
+    if (lcf) f |= JF_LOAD_CFAST;
+    const bool scf = J.o_c == 1 && J.o_lbits >= 2;
+    if (scf) f |= JF_STORE_CFAST;
+    const int64_t ib = (int64_t)sbytes(sin), ob = (int64_t)sbytes(sout);
+    bool vl = aligned16(J.in) && (J.in_pstride * ib) % 16 == 0;
+    if (lcf) vl = vl && C >= 4 && (J.s_r % 4) == 0 && (J.ncols % 4) == 0;
+    else vl = vl && J.s_r == 1 && R >= 4 && (J.s_c % 4) == 0;
+    if (vl) f |= JF_VEC4;
+    bool vs = aligned16(J.out) && (J.out_pstride * ob) % 16 == 0;
+    if (scf) vs = vs && C >= 4 && (J.o_r % 4) == 0 && (J.o_hs % 4) == 0 && (J.ncols % 4) == 0;
+    else vs = vs && J.o_r == 1 && R >= 4 && (J.o_hs % 4) == 0 && (J.o_c % 4 == 0 || J.o_lbits == 0);
+    if (vs) f |= JF_VEC4_ST;
+    const uint32_t tile = (R << J.log_c) * vbytes(ar);
+    if (tile + R * 2 * vbytes(ar) <= kLdsMax) f |= JF_Z_LDS;
+    J.flags = f;
+}
+
+static uint32_t job_lds(const PassJob& J, Arith ar) {
+    const uint32_t R = 1u << J.log_r;
+    uint32_t b = (R << J.log_c) * vbytes(ar);
+    if (J.flags & JF_Z_LDS) b += R * 2 * vbytes(ar);
+    return b;
+}
 
 // Fills the PassJobs of `npoly` polynomials of plan P at data (stride n
-// elements), with four-step scratch at `scratch` (stride n, arithmetic width).
-static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, PassJob out[2]) {
-    const uint64_t N = 1ull << P.log_n;
+// elements), with workspace at `scratch` (stride n, arithmetic width).
+static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, PassJob out[4],
+                      Store sin_out[4][2]) {
+    const uint32_t log_n = P.log_n;
+    const int64_t N = (int64_t)1 << log_n;
     const char* blk = (const char*)P.block;
+    const Store mid = P.arith == Arith::M32 ? Store::U32 : Store::U64;
+    void* src = data;
+    Store src_st = st;
     for (int i = 0; i < P.npass; i++) {
         const PassSpec& s = P.pass[i];
         PassJob& J = out[i];
         memset(&J, 0, sizeof(J));
         J.z = blk + s.z_off;
-        J.post_lo = s.lo_off >= 0 ? blk + s.lo_off : nullptr;
-        J.post_hi = s.hi_off >= 0 ? blk + s.hi_off : nullptr;
+        J.tw_lo = s.lo_off >= 0 ? blk + s.lo_off : nullptr;
+        J.tw_hi = s.hi_off >= 0 ? blk + s.hi_off : nullptr;
         J.q = P.q;
         J.scale_w = s.scale_w;
         J.scale_wp = s.scale_wp;
         J.log_r = s.log_r;
         J.post = s.post;
-        J.e_a = s.e_a;
-        J.e_b = s.e_b;
-        J.e_d = s.e_d;
-        J.e_mask = s.e_mask;
-        J.lo_bits = s.lo_bits;
+        J.e_a = s.e_a; J.e_b = s.e_b; J.e_d = s.e_d;
+        J.e_shift = s.e_shift; J.e_mask = s.e_mask; J.lo_bits = s.lo_bits;
+        const bool last = i == P.npass - 1;
+        void* dst = last ? data : (src == data ? scratch : data);
+        const Store dst_st = last ? st : (src == data ? mid : st);
+        J.in = src;
+        J.out = dst;
+        sin_out[i][0] = src_st;
+        sin_out[i][1] = dst_st;
         if (P.npass == 1) {
             // columns are polynomials: element (r, p) at p*N + r
-            J.in = data;
-            J.out = data;
-            J.s_r = 1; J.s_c = (int64_t)N;
-            J.o_r = 1; J.o_c = (int64_t)N;
-            const uint32_t lc = (uint32_t)std::min<uint64_t>(s.log_c, 63 - __builtin_clzll(npoly | 1));
-            J.log_c = (npoly >= (1ull << s.log_c)) ? s.log_c : lc;
+            J.s_r = 1; J.s_c = N;
+            J.o_r = 1; J.o_c = 0; J.o_hs = N; J.o_lbits = 0;
+            const uint32_t lc = std::min<uint32_t>(s.log_c, ilog2_floor(npoly));
+            J.log_c = lc;
             J.ncols = (uint32_t)npoly;
-            J.tiles_per_poly = (uint32_t)((npoly + (1ull << J.log_c) - 1) >> J.log_c);
+            J.tiles_per_poly = (uint32_t)((npoly + (1ull << lc) - 1) >> lc);
             J.npoly = 1;
             J.in_pstride = J.out_pstride = 0;
         } else {
-            const uint64_t N1 = 1ull << P.log_n1, N2 = 1ull << P.log_n2;
+            const uint32_t lr = s.log_r, ll = s.log_l;
             J.log_c = s.log_c;
-            J.npoly = (uint32_t)npoly;
-            if (i == 0) {  // data [N1][N2] -> scratch [N2][N1] (transposed)
-                J.in = data; J.out = scratch;
-                J.s_r = (int64_t)N2; J.s_c = 1;
-                J.o_r = 1; J.o_c = (int64_t)N1;
-                J.ncols = (uint32_t)N2;
-            } else {       // scratch [N2][N1] -> data (k2, k1) at k2*N1 + k1
-                J.in = scratch; J.out = data;
-                J.s_r = (int64_t)N1; J.s_c = 1;
-                J.o_r = (int64_t)N1; J.o_c = 1;
-                J.ncols = (uint32_t)N1;
-            }
+            J.s_r = N >> lr; J.s_c = 1;
+            if (last) { J.o_lbits = log_n - lr; J.o_hs = 0; J.o_c = 1; J.o_r = (int64_t)1 << ll; }
+            else { J.o_lbits = ll; J.o_hs = (int64_t)1 << (lr + ll); J.o_c = 1; J.o_r = (int64_t)1 << ll; }
+            J.ncols = (uint32_t)(N >> lr);
             J.tiles_per_poly = J.ncols >> J.log_c;
-            J.in_pstride = J.out_pstride = (int64_t)N;
+            J.npoly = (uint32_t)npoly;
+            J.in_pstride = J.out_pstride = N;
         }
+        finish_flags(J, P.arith, src_st, dst_st);
+        src = dst;
+        src_st = dst_st;
     }
-    (void)st;
 }
 
 static uint32_t job_tiles(const PassJob& J) { return J.tiles_per_poly * (J.npoly ? J.npoly : 1); }
@@ -336,14 +416,14 @@ static uint32_t job_tiles(const PassJob& J) { return J.tiles_per_poly * (J.npoly
 template <class M, class Sin, class Sout>
 static ntt_status launch_t(const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t nblocks, uint32_t lds,
                            hipStream_t stream) {
-    auto fn = pass_kernel<M, Sin, Sout, 4>;
+    auto fn = pass_kernel<M, Sin, Sout, 4, kThreads>;
     if (lds > 65536) {
         static std::once_flag once;
         std::call_once(once, [&] {
-            (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, device_max_lds());
+            (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
         });
     }
-    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(256), lds, stream, djobs, njobs, inl);
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(kThreads), lds, stream, djobs, njobs, inl);
     HIP_TRY(hipGetLastError());
     return NTT_OK;
 }
@@ -361,17 +441,17 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, const PassJob* dj
     return launch_t<Mod64, uint64_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
 }
 
-static uint32_t job_lds(const PassJob& J, Arith ar) { return (1u << (J.log_r + J.log_c)) * vbytes(ar); }
-
 // Runs plan P on npoly polynomials at data (contiguous, stride n).
 static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, hipStream_t stream) {
-    PassJob jobs[2];
-    make_jobs(P, data, st, npoly, scratch, jobs);
-    const Store mid = P.arith == Arith::M32 ? Store::U32 : Store::U64;
-    if (P.npass == 1) return launch_pass(P.arith, st, st, nullptr, 0, jobs[0], job_tiles(jobs[0]), job_lds(jobs[0], P.arith), stream);
-    ntt_status s = launch_pass(P.arith, st, mid, nullptr, 0, jobs[0], job_tiles(jobs[0]), job_lds(jobs[0], P.arith), stream);
-    if (s != NTT_OK) return s;
-    return launch_pass(P.arith, mid, st, nullptr, 0, jobs[1], job_tiles(jobs[1]), job_lds(jobs[1], P.arith), stream);
+    PassJob jobs[4];
+    Store io[4][2];
+    make_jobs(P, data, st, npoly, scratch, jobs, io);
+    for (int i = 0; i < P.npass; i++) {
+        ntt_status s = launch_pass(P.arith, io[i][0], io[i][1], nullptr, 0, jobs[i], job_tiles(jobs[i]),
+                                   job_lds(jobs[i], P.arith), stream);
+        if (s != NTT_OK) return s;
+    }
+    return NTT_OK;
 }
 
 // ------------------------------------------------------------------ config (include/ntt.cuh)
@@ -450,7 +530,7 @@ extern "C" ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out) {
     for (int nega = 0; nega < 2; nega++)
         for (int inv = 0; inv < 2; inv++) {
             const int k = nega * 2 + inv;
-            I->plan_status[k] = build_plan(q, log_n, nega, inv, I->arith, I->plans[k]);
+            I->plan_status[k] = build_plan(q, log_n, nega, inv, I->arith, choose_split(log_n, I->arith, n), I->plans[k]);
             if (I->plan_status[k] != NTT_OK) {
                 I->plan_error[k] = g_last_error;
                 if (!nega) return I->plan_status[k];  // cyclic must exist
@@ -650,7 +730,7 @@ extern "C" ntt_status ntt_c_host(uint64_t* data, uint64_t n, const ntt_c_modular
 // ------------------------------------------------------------------ context + descriptor batches
 struct ntt_c_context {
     std::mutex mu;
-    std::map<std::tuple<uint64_t, uint32_t, int, int, int>, std::unique_ptr<Plan>> plans;
+    std::map<std::tuple<uint64_t, uint32_t, int, int, int, Split>, std::unique_ptr<Plan>> plans;
     uint64_t table_bytes = 0;
     std::mutex smu;
     std::map<hipStream_t, std::pair<void*, uint64_t>> scratch;  // per-call four-step workspace per stream
@@ -665,15 +745,19 @@ static Arith arith_for(uint64_t q, uint32_t flags) {
     return q < (1ull << 31) ? Arith::M32 : Arith::M64;
 }
 
-static ntt_status ctx_plan(ntt_c_context* ctx, const ntt_c_desc& d, const Plan** out) {
+// Plan for a descriptor; `elems_in_flight` (elements transformed by the same
+// launches) steers the radix split, or `forced` overrides it.
+static ntt_status ctx_plan(ntt_c_context* ctx, const ntt_c_desc& d, uint64_t elems_in_flight, const Plan** out,
+                           const Split* forced = nullptr) {
     const bool nega = d.flags & NTT_FLAG_NEGACYCLIC, inv = d.flags & NTT_FLAG_INVERSE;
     const Arith ar = arith_for(d.q, d.flags);
-    auto key = std::make_tuple(d.q, d.log_n, (int)nega, (int)inv, (int)ar);
+    const Split sp = forced ? *forced : choose_split(d.log_n, ar, elems_in_flight);
+    auto key = std::make_tuple(d.q, d.log_n, (int)nega, (int)inv, (int)ar, sp);
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->plans.find(key);
     if (it == ctx->plans.end()) {
         std::unique_ptr<Plan> P;
-        ntt_status st = build_plan(d.q, d.log_n, nega, inv, ar, P);
+        ntt_status st = build_plan(d.q, d.log_n, nega, inv, ar, sp, P);
         if (st != NTT_OK) return st;
         ctx->table_bytes += P->bytes;
         it = ctx->plans.emplace(key, std::move(P)).first;
@@ -714,7 +798,7 @@ struct ntt_c_batch {
         PassJob* djobs = nullptr;
         PassJob inl{};
     };
-    std::vector<Group> phases[2];  // phase 0: single-pass + pass 1; phase 1: pass 2
+    std::vector<Group> phases[4];  // phase k: pass k of every polynomial
     void* scratch = nullptr;
     uint64_t scratch_bytes = 0;
     uint64_t workgroups = 0;
@@ -738,11 +822,14 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
     struct Item { PassJob job; Arith ar; Store sin, sout; int phase; };
     std::vector<Item> items;
     std::vector<const Plan*> plans(count);
-    uint64_t scratch_bytes = 0;
+    uint64_t scratch_bytes = 0, elems = 0;
     for (size_t i = 0; i < count; i++) {
         ntt_status st = check_desc(descs[i]);
         if (st != NTT_OK) return st;
-        st = ctx_plan(ctx, descs[i], &plans[i]);
+        elems += 1ull << descs[i].log_n;
+    }
+    for (size_t i = 0; i < count; i++) {
+        ntt_status st = ctx_plan(ctx, descs[i], elems, &plans[i]);
         if (st != NTT_OK) return st;
         scratch_bytes += ((plan_scratch_elems(*plans[i]) * vbytes(plans[i]->arith)) + 255) & ~255ull;
     }
@@ -755,20 +842,15 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
     for (size_t i = 0; i < count; i++) {
         const Plan& P = *plans[i];
         const Store st = (descs[i].flags & NTT_FLAG_U64) ? Store::U64 : Store::U32;
-        const Store mid = P.arith == Arith::M32 ? Store::U32 : Store::U64;
         void* scr = nullptr;
-        if (P.npass == 2) {
+        if (P.npass >= 2) {
             scr = (char*)B->scratch + soff;
             soff += ((plan_scratch_elems(P) * vbytes(P.arith)) + 255) & ~255ull;
         }
-        PassJob jobs[2];
-        make_jobs(P, descs[i].data, st, 1, scr, jobs);
-        if (P.npass == 1) {
-            items.push_back({jobs[0], P.arith, st, st, 0});
-        } else {
-            items.push_back({jobs[0], P.arith, st, mid, 0});
-            items.push_back({jobs[1], P.arith, mid, st, 1});
-        }
+        PassJob jobs[4];
+        Store io[4][2];
+        make_jobs(P, descs[i].data, st, 1, scr, jobs, io);
+        for (int k = 0; k < P.npass; k++) items.push_back({jobs[k], P.arith, io[k][0], io[k][1], k});
     }
     // Group by (phase, arith, storage); within a group order by LDS size so
     // that big tiles start first.
@@ -780,6 +862,7 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
         size_t j = i;
         ntt_c_batch::Group g;
         g.ar = items[i].ar;
+        memset(&g.inl, 0, sizeof(g.inl));
         g.sin = items[i].sin;
         g.sout = items[i].sout;
         std::vector<PassJob> jobs;
@@ -855,39 +938,68 @@ extern "C" ntt_status ntt_c_batch_destroy(ntt_c_batch* B) {
 
 extern "C" ntt_status ntt_c_batch_info(const ntt_c_batch* B, uint32_t* launches, uint64_t* ws, uint64_t* wgs) {
     if (!B) return fail(NTT_ERR_INVALID_ARG, "batch_info: null batch");
-    if (launches) *launches = (uint32_t)(B->phases[0].size() + B->phases[1].size());
+    if (launches) {
+        uint32_t l = 0;
+        for (auto& ph : B->phases) l += (uint32_t)ph.size();
+        *launches = l;
+    }
     if (ws) *ws = B->scratch_bytes;
     if (wgs) *wgs = B->workgroups;
     return NTT_OK;
 }
 
-extern "C" ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* d, void* stream) {
+static ntt_status ctx_scratch(ntt_c_context* ctx, hipStream_t stream, uint64_t need, void** out) {
+    *out = nullptr;
+    if (!need) return NTT_OK;
+    std::lock_guard<std::mutex> lk(ctx->smu);
+    auto& ent = ctx->scratch[stream];
+    if (ent.second < need) {
+        if (ent.first) {
+            HIP_TRY(hipStreamSynchronize(stream));
+            HIP_TRY(hipFree(ent.first));
+        }
+        ent.first = nullptr;
+        ent.second = 0;
+        if (hipMalloc(&ent.first, need) != hipSuccess) return fail(NTT_ERR_OOM, "transform: scratch allocation failed");
+        ent.second = need;
+    }
+    *out = ent.first;
+    return NTT_OK;
+}
+
+static ntt_status transform_one(ntt_c_context* ctx, const ntt_c_desc* d, const Split* forced, void* stream) {
     if (!ctx || !d) return fail(NTT_ERR_INVALID_ARG, "transform: null argument");
     ntt_status st = check_desc(*d);
     if (st != NTT_OK) return st;
     const Plan* P = nullptr;
-    st = ctx_plan(ctx, *d, &P);
+    st = ctx_plan(ctx, *d, 1ull << d->log_n, &P, forced);
     if (st != NTT_OK) return st;
-    // Per-call path: the four-step workspace comes from a context-owned
-    // per-stream buffer (no allocation once it is large enough).
+    // The four-step workspace is a context-owned per-stream buffer (no
+    // allocation once it is large enough).
     void* scr = nullptr;
-    const uint64_t need = plan_scratch_elems(*P) * vbytes(P->arith);
-    if (need) {
-        std::lock_guard<std::mutex> lk(ctx->smu);
-        auto& ent = ctx->scratch[(hipStream_t)stream];
-        if (ent.second < need) {
-            if (ent.first) {
-                HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
-                HIP_TRY(hipFree(ent.first));
-            }
-            ent.first = nullptr;
-            ent.second = 0;
-            if (hipMalloc(&ent.first, need) != hipSuccess) return fail(NTT_ERR_OOM, "transform: scratch allocation failed");
-            ent.second = need;
-        }
-        scr = ent.first;
-    }
+    st = ctx_scratch(ctx, (hipStream_t)stream, plan_scratch_elems(*P) * vbytes(P->arith), &scr);
+    if (st != NTT_OK) return st;
     return run_plan(*P, d->data, (d->flags & NTT_FLAG_U64) ? Store::U64 : Store::U32, 1, scr, (hipStream_t)stream);
+}
+
+extern "C" ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* d, void* stream) {
+    return transform_one(ctx, d, nullptr, stream);
+}
+
+extern "C" ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc* d, uint32_t npass,
+                                            const uint32_t* log_r, uint32_t log_e, void* stream) {
+    if (!log_r || npass < 1 || npass > 4) return fail(NTT_ERR_INVALID_ARG, "transform_split: bad split");
+    Split sp;
+    sp.npass = npass;
+    for (uint32_t i = 0; i < npass; i++) sp.log_r[i] = log_r[i];
+    sp.log_e = log_e;
+    if (d) {
+        const uint32_t emax = max_log_e(arith_for(d->q, d->flags));
+        for (uint32_t i = 0; i < npass; i++)
+            if (log_r[i] < 1 || log_r[i] > emax) return fail(NTT_ERR_INVALID_ARG, "transform_split: radix too large");
+        if (log_e > emax) return fail(NTT_ERR_INVALID_ARG, "transform_split: tile too large");
+    }
+    return transform_one(ctx, d, &sp, stream);
 }
 
 // ------------------------------------------------------------------ device helpers

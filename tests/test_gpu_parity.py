@@ -54,6 +54,8 @@ def test_cfg1_babybear_4096_all_variants(nt, orc, cfgs, variant):
 @pytest.mark.parametrize("q", [BB, P30])
 def test_cyclic_vs_oracle(nt, orc, cfgs, log_n, q):
     n = 1 << log_n
+    if (q - 1) % n:
+        pytest.skip("n does not divide q-1")
     c = cfgs(n, q)
     x = orc.random_poly(n, q, 1000 + log_n)
     assert np.array_equal(gpu_apply(nt, x, c.forward), orc.ntt(x, q))
@@ -64,6 +66,8 @@ def test_cyclic_vs_oracle(nt, orc, cfgs, log_n, q):
 @pytest.mark.parametrize("q", [BB, P30])
 def test_negacyclic_vs_oracle(nt, orc, cfgs, log_n, q):
     n = 1 << log_n
+    if (q - 1) % (2 * n):
+        pytest.skip("2n does not divide q-1")
     c = cfgs(n, q)
     x = orc.random_poly(n, q, 2000 + log_n)
     assert np.array_equal(gpu_apply(nt, x, c.forward_negacyclic), orc.ntt_negacyclic(x, q))

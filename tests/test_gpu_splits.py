@@ -1,0 +1,59 @@
+"""Every plan shape (1..4 passes, uneven radices, narrow and wide tiles,
+uint32 and uint64 storage, 31- and 62-bit primes) against the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BB = 2013265921
+P62 = 4611686018326724609
+
+SPLITS = [
+    (6, [6], 6), (6, [6], 12), (8, [4, 4], 8), (8, [5, 3], 10), (9, [3, 3, 3], 9), (10, [4, 3, 3], 14),
+    (10, [2, 2, 3, 3], 11), (12, [12], 14), (13, [7, 6], 11), (14, [14], 14), (16, [8, 8], 12),
+    (16, [6, 5, 5], 11), (17, [6, 6, 5], 14), (18, [9, 9], 14), (20, [10, 10], 14), (20, [7, 7, 6], 12),
+    (20, [5, 5, 5, 5], 11),
+]
+
+
+@pytest.fixture(scope="module")
+def ctx(nt):
+    c = nt.Context()
+    yield c
+    c.destroy()
+
+
+@pytest.mark.parametrize("log_n,split,log_e", SPLITS)
+@pytest.mark.parametrize("kind", ["cyc", "icyc", "nega", "inega"])
+def test_split_u32(nt, orc, ctx, log_n, split, log_e, kind):
+    n = 1 << log_n
+    x = orc.random_poly(n, BB, 7 * log_n + len(split))
+    flags = {"cyc": 0, "icyc": nt.FLAG_INVERSE, "nega": nt.FLAG_NEGACYCLIC,
+             "inega": nt.FLAG_NEGACYCLIC | nt.FLAG_INVERSE}[kind]
+    buf = nt.DeviceBuffer.from_array(x.astype(np.uint32))
+    ctx.transform_split(buf.ptr, BB, log_n, flags, split, log_e)
+    nt.device_sync()
+    got = buf.download(np.uint32, n).astype(np.uint64)
+    assert np.array_equal(got, orc.transform(x, BB, "nega" in kind, kind.startswith("i")))
+
+
+@pytest.mark.parametrize("log_n,split,log_e", [(8, [4, 4], 8), (12, [6, 6], 10), (13, [5, 4, 4], 13),
+                                               (16, [8, 8], 12), (18, [6, 6, 6], 12)])
+@pytest.mark.parametrize("q", [BB, P62])
+def test_split_u64(nt, orc, ctx, log_n, split, log_e, q):
+    n = 1 << log_n
+    x = orc.random_poly(n, q, 3 * log_n)
+    for flags, nega, inv in [(0, False, False), (3, True, True), (2, True, False)]:
+        buf = nt.DeviceBuffer.from_array(x)
+        ctx.transform_split(buf.ptr, q, log_n, flags | nt.FLAG_U64, split, min(log_e, 13 if q == P62 else 14))
+        nt.device_sync()
+        assert np.array_equal(buf.download(np.uint64, n), orc.transform(x, q, nega, inv)), (flags, split)
+        buf.free()
+
+
+def test_split_rejects_bad_shapes(nt, ctx):
+    buf = nt.DeviceBuffer(1 << 12)
+    with pytest.raises(nt.NttError):
+        ctx.transform_split(buf.ptr, BB, 10, 0, [4, 4], 12)      # does not cover n
+    with pytest.raises(nt.NttError):
+        ctx.transform_split(buf.ptr, BB, 10, 0, [10], 15)        # tile too large
