@@ -197,6 +197,43 @@ def extra_u64_abi(args, stream):
             "frac": 2 * n * 8 / t / 1e9 / HBM_PEAK_GBS}
 
 
+def extra_serving(ctx, qps_list=(50, 100, 200, 400, 800), count=100):
+    """cfg5: Poisson arrivals, 1..32 polys per request, log2 n in [12, 22],
+    BabyBear (uint32) or a 62-bit prime (uint64); device-resident operands."""
+    base = W.serving_requests(count, 1.0, seed=5)
+    sizes = []
+    for _, polys in base:
+        sizes.append([(1 << p["log_n"]) * (8 if p["q"] > (1 << 31) else 4) for p in polys])
+    total = sum((b + 255) // 256 * 256 for r in sizes for b in r)
+    pool = nt.DeviceBuffer(total)
+    nt.check(nt.lib().ntt_c_memset(pool.ptr, 1, total))  # 0x01.. < q for both prime kinds
+    reqs, off = [], 0
+    for (_, polys), szs in zip(base, sizes):
+        r = []
+        for p, b in zip(polys, szs):
+            u64 = p["q"] > (1 << 31)
+            flags = nt.FLAG_NEGACYCLIC | (nt.FLAG_INVERSE if p["inverse"] else 0) | (nt.FLAG_U64 if u64 else 0)
+            r.append((pool.ptr + off, p["q"], p["log_n"], flags))
+            off += (b + 255) // 256 * 256
+        reqs.append(r)
+    bfly = sum(W.butterflies(p["log_n"]) for _, polys in base for p in polys)
+    srv = nt.Server(ctx, hbm_budget_bytes=16 << 30, max_inflight=4, max_batch_polys=1024)
+    srv.replay(reqs, [0.0] * len(reqs))  # builds every plan once (twiddles resident)
+    out = {"requests": count, "polys": sum(len(r) for r in reqs), "butterflies": bfly, "points": []}
+    for qps in qps_list:
+        srv.reset_stats()
+        arrivals = [t / qps for t, _ in base]
+        lat = srv.replay(reqs, arrivals)
+        st = srv.stats()
+        out["points"].append({"qps": qps, "p50_us": float(np.percentile(lat, 50)),
+                              "p99_us": float(np.percentile(lat, 99)), "Gbfly_s": bfly / st["wall_s"] / 1e9,
+                              "batches": st["batches"], "max_concurrent_batches": st["max_concurrent_batches"]})
+        log(f"[bench]   qps={qps}: p50 {out['points'][-1]['p50_us']:.0f} us p99 {out['points'][-1]['p99_us']:.0f} us")
+    srv.destroy()
+    pool.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,7 +261,8 @@ def main():
                          ("negacyclic_2_20", lambda: extra_2_20(ctx, stream)),
                          ("rns_24x2_16", lambda: extra_rns(ctx, stream)),
                          ("uneven_512", lambda: extra_uneven(ctx, stream)),
-                         ("u64_abi", lambda: extra_u64_abi(args, stream))]:
+                         ("u64_abi", lambda: extra_u64_abi(args, stream)),
+                         ("serving", lambda: extra_serving(ctx))]:
             log(f"[bench] extra {name}")
             extras[name] = fn()
     cpu = None

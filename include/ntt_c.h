@@ -170,6 +170,42 @@ ntt_status ntt_c_batch_destroy(ntt_c_batch* batch);
 ntt_status ntt_c_batch_info(const ntt_c_batch* batch, uint32_t* launches, uint64_t* workspace_bytes,
                             uint64_t* workgroups);
 
+/* ---------------- continuous-batching server (no reference counterpart) ----------------
+ * Requests are lists of device-resident polynomials (ntt_c_desc, transformed in
+ * place).  A scheduler thread admits queued requests in arrival order while
+ * their four-step workspace fits the HBM budget, packs every admitted request
+ * into one batch launch per free slot (stream), and overlaps up to
+ * max_inflight batches on the device.  Twiddles come from the context's
+ * device-resident plan cache.  Latency = submit -> completion seen by the host. */
+typedef struct ntt_c_server ntt_c_server;
+typedef struct {
+    uint64_t hbm_budget_bytes;  /* arena for workspace (default 1 GiB) */
+    uint32_t max_inflight;      /* concurrent batches / streams (1..16, default 4) */
+    uint32_t max_batch_polys;   /* polynomials per batch (default 1024) */
+    int device;                 /* set by create: the current HIP device */
+} ntt_c_server_opts;
+typedef struct {
+    uint64_t completed, failed, batches, butterflies;
+    double p50_us, p99_us, mean_us;
+    double wall_s;                  /* first submit -> last completion */
+    uint32_t max_concurrent_batches;
+    uint64_t arena_free_bytes;
+} ntt_c_server_stats;
+
+ntt_status ntt_c_server_create(ntt_c_context* ctx, const ntt_c_server_opts* opts, ntt_c_server** out);
+/* Queues one request; NTT_ERR_OOM if it can never fit the budget. */
+ntt_status ntt_c_server_submit(ntt_c_server* srv, const ntt_c_desc* polys, size_t count, uint64_t* request_id);
+/* Blocks until the request finished (timeout_s < 0: no timeout; NTT_ERR_BUSY on timeout). */
+ntt_status ntt_c_server_wait(ntt_c_server* srv, uint64_t request_id, double timeout_s);
+ntt_status ntt_c_server_poll(ntt_c_server* srv, uint64_t request_id, int* done, double* latency_us);
+ntt_status ntt_c_server_get_stats(ntt_c_server* srv, ntt_c_server_stats* out);
+ntt_status ntt_c_server_reset_stats(ntt_c_server* srv);
+/* Submits request i at arrival_s[i] seconds after the call (polys packed
+ * back to back, counts[i] per request), waits for all, writes latencies. */
+ntt_status ntt_c_server_replay(ntt_c_server* srv, const ntt_c_desc* polys, const uint32_t* counts,
+                               const double* arrival_s, size_t nreq, double* latency_us);
+ntt_status ntt_c_server_destroy(ntt_c_server* srv);
+
 /* ---------------- device-memory and timing helpers for hosts without a HIP
  * binding (tests, benchmarks, FFI callers) ---------------- */
 ntt_status ntt_c_device_count(int* count);

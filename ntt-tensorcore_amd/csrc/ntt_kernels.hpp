@@ -288,29 +288,52 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job,
     };
     if (flags & JF_VEC4_ST) {
         const uint32_t nq = total >> 2;
-        for (uint32_t qi = tid; qi < nq; qi += T) {
-            const uint32_t idx = qi << 2;
-            uint32_t k, c;
-            if (flags & JF_STORE_CFAST) { k = idx >> log_c; c = idx & (C - 1u); }
-            else { c = idx >> log_r; k = idx & (R - 1u); }
-            if (col0 + c >= ncols) continue;
-            Sout v[4];
+        for (uint32_t q0 = tid; q0 < nq; q0 += kUnroll * T) {
+            // Phase 1: positions, LDS reads and post-multiply for kUnroll quads;
+            // phase 2: the stores.  Keeps the table loads of all quads in flight.
+            Sout v[kUnroll][4];
+            int64_t a[kUnroll];
+            bool ok[kUnroll];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const uint32_t kk = k + ((flags & JF_STORE_CFAST) ? 0u : (uint32_t)e);
-                const uint32_t ce = c + ((flags & JF_STORE_CFAST) ? (uint32_t)e : 0u);
-                v[e] = (Sout)post(lds[swz((bitrev_bits(kk, log_r) << log_c) + ce)], kk, col0 + ce);
+            for (int u = 0; u < kUnroll; u++) {
+                const uint32_t qi = q0 + u * T;
+                const uint32_t idx = qi << 2;
+                uint32_t k, c;
+                if (flags & JF_STORE_CFAST) { k = idx >> log_c; c = idx & (C - 1u); }
+                else { c = idx >> log_r; k = idx & (R - 1u); }
+                ok[u] = qi < nq && col0 + c < ncols;
+                a[u] = oaddr(k, col0 + c);
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const uint32_t kk = k + ((flags & JF_STORE_CFAST) ? 0u : (uint32_t)e);
+                    const uint32_t ce = c + ((flags & JF_STORE_CFAST) ? (uint32_t)e : 0u);
+                    v[u][e] = ok[u] ? (Sout)post(lds[swz((bitrev_bits(kk & (R - 1u), log_r) << log_c) + (ce & (C - 1u)))],
+                                                 kk, col0 + ce)
+                                    : (Sout)0;
+                }
             }
-            store4<Sout>(out + oaddr(k, col0 + c), v);
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+                if (ok[u]) store4<Sout>(out + a[u], v[u]);
         }
     } else {
-        for (uint32_t idx = tid; idx < total; idx += T) {
-            uint32_t k, c;
-            if (flags & JF_STORE_CFAST) { k = idx >> log_c; c = idx & (C - 1u); }
-            else { c = idx >> log_r; k = idx & (R - 1u); }
-            if (col0 + c >= ncols) continue;
-            const V v = post(lds[swz((bitrev_bits(k, log_r) << log_c) + c)], k, col0 + c);
-            out[oaddr(k, col0 + c)] = (Sout)v;
+        for (uint32_t i0 = tid; i0 < total; i0 += kUnroll * T) {
+            V v[kUnroll];
+            int64_t a[kUnroll];
+            bool ok[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const uint32_t idx = i0 + u * T;
+                uint32_t k, c;
+                if (flags & JF_STORE_CFAST) { k = idx >> log_c; c = idx & (C - 1u); }
+                else { c = idx >> log_r; k = idx & (R - 1u); }
+                ok[u] = idx < total && col0 + c < ncols;
+                a[u] = oaddr(k, col0 + c);
+                v[u] = ok[u] ? post(lds[swz((bitrev_bits(k, log_r) << log_c) + c)], k, col0 + c) : (V)0;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+                if (ok[u]) out[a[u]] = (Sout)v[u];
         }
     }
 }

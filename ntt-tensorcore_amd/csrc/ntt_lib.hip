@@ -1002,6 +1002,8 @@ extern "C" ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc
     return transform_one(ctx, d, &sp, stream);
 }
 
+#include "ntt_serve.inc"
+
 // ------------------------------------------------------------------ device helpers
 extern "C" ntt_status ntt_c_device_count(int* c) {
     if (!c) return fail(NTT_ERR_INVALID_ARG, "null");

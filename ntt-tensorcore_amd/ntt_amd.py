@@ -344,3 +344,84 @@ class Batch:
         if self.ptr:
             check(lib().ntt_c_batch_destroy(self.ptr))
             self.ptr = None
+
+
+# ---------------------------------------------------------------- serving
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_uint32),
+                ("max_batch_polys", ctypes.c_uint32), ("device", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("butterflies", ctypes.c_uint64), ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double),
+                ("mean_us", ctypes.c_double), ("wall_s", ctypes.c_double),
+                ("max_concurrent_batches", ctypes.c_uint32), ("arena_free_bytes", ctypes.c_uint64)]
+
+
+def _server_sigs(L):
+    vp, u32, u64, P = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER
+    for name, args in {
+        "ntt_c_server_create": [vp, P(ServerOpts), P(vp)],
+        "ntt_c_server_submit": [vp, P(Desc), ctypes.c_size_t, P(u64)],
+        "ntt_c_server_wait": [vp, u64, ctypes.c_double],
+        "ntt_c_server_poll": [vp, u64, P(ctypes.c_int), P(ctypes.c_double)],
+        "ntt_c_server_get_stats": [vp, P(ServerStats)],
+        "ntt_c_server_reset_stats": [vp],
+        "ntt_c_server_replay": [vp, P(Desc), P(u32), P(ctypes.c_double), ctypes.c_size_t, P(ctypes.c_double)],
+        "ntt_c_server_destroy": [vp],
+    }.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+
+
+_EXTRA_SIGS.append(_server_sigs)
+
+
+class Server:
+    """Continuous-batching server over device-resident polynomials."""
+
+    def __init__(self, ctx, hbm_budget_bytes=1 << 30, max_inflight=4, max_batch_polys=1024):
+        o = ServerOpts(hbm_budget_bytes, max_inflight, max_batch_polys, 0)
+        p = ctypes.c_void_p()
+        check(lib().ntt_c_server_create(ctx.ptr, ctypes.byref(o), ctypes.byref(p)))
+        self.ptr = p.value
+
+    def submit(self, polys):
+        """polys: list of (data_ptr, q, log_n, flags); returns the request id."""
+        arr = (Desc * len(polys))(*[Desc(*d) for d in polys])
+        rid = ctypes.c_uint64()
+        check(lib().ntt_c_server_submit(self.ptr, arr, len(polys), ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, timeout_s=-1.0):
+        check(lib().ntt_c_server_wait(self.ptr, rid, timeout_s))
+
+    def poll(self, rid):
+        done, lat = ctypes.c_int(), ctypes.c_double()
+        check(lib().ntt_c_server_poll(self.ptr, rid, ctypes.byref(done), ctypes.byref(lat)))
+        return bool(done.value), lat.value
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().ntt_c_server_get_stats(self.ptr, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        check(lib().ntt_c_server_reset_stats(self.ptr))
+
+    def replay(self, requests, arrivals_s):
+        """requests: list of lists of (ptr, q, log_n, flags); returns latencies (us)."""
+        flat = [d for r in requests for d in r]
+        arr = (Desc * len(flat))(*[Desc(*d) for d in flat])
+        counts = (ctypes.c_uint32 * len(requests))(*[len(r) for r in requests])
+        arrivals = (ctypes.c_double * len(requests))(*arrivals_s)
+        lat = (ctypes.c_double * len(requests))()
+        check(lib().ntt_c_server_replay(self.ptr, arr, counts, arrivals, len(requests), lat))
+        return list(lat)
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().ntt_c_server_destroy(self.ptr))
+            self.ptr = None
