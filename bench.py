@@ -197,7 +197,7 @@ def extra_u64_abi(args, stream):
             "frac": 2 * n * 8 / t / 1e9 / HBM_PEAK_GBS}
 
 
-def extra_serving(ctx, qps_list=(50, 100, 200, 400, 800), count=100):
+def extra_serving(ctx, qps_list=(100, 500, 1000, 2000, 4000, 8000), count=120):
     """cfg5: Poisson arrivals, 1..32 polys per request, log2 n in [12, 22],
     BabyBear (uint32) or a 62-bit prime (uint64); device-resident operands."""
     base = W.serving_requests(count, 1.0, seed=5)

@@ -74,6 +74,7 @@ enum JobFlags : uint32_t {
     JF_VEC4 = 4u,         // 4 consecutive elements per thread on the load side (aligned)
     JF_Z_LDS = 8u,        // stage the CT table in LDS
     JF_VEC4_ST = 16u,     // 4 consecutive elements per thread on the store side
+    JF_MFMA = 32u,        // pass runs on the matrix cores (ntt_mfma.hpp)
 };
 
 struct PassJob {

@@ -21,6 +21,7 @@
 #include "../../include/ntt_c.h"
 #include "field_host.hpp"
 #include "ntt_kernels.hpp"
+#include "ntt_mfma.hpp"
 
 using namespace nttk;
 using namespace ntth;
@@ -100,8 +101,7 @@ __global__ void powers_kernel(uint64_t* out, uint64_t n, const TwP<typename M::V
 }  // namespace nttk
 
 // ------------------------------------------------------------------ device limits
-// LDS budget of one workgroup's tile (the CT table may add up to 16 KB).
-static const uint32_t kTileBytes = 65536;
+// LDS budget of one workgroup (tile plus, when it fits, the CT table).
 static const uint32_t kLdsMax = 81920;
 static const int kThreads = 512;
 
@@ -117,9 +117,10 @@ struct Split {
     uint32_t npass = 1;
     uint32_t log_r[4] = {0, 0, 0, 0};
     uint32_t log_e = 14;
+    uint32_t mfma = 0;  // bit i: pass i runs on the INT8 matrix cores (R = 64 or 4096)
     bool operator<(const Split& o) const {
-        return std::make_tuple(npass, log_r[0], log_r[1], log_r[2], log_r[3], log_e) <
-               std::make_tuple(o.npass, o.log_r[0], o.log_r[1], o.log_r[2], o.log_r[3], o.log_e);
+        return std::make_tuple(npass, log_r[0], log_r[1], log_r[2], log_r[3], log_e, mfma) <
+               std::make_tuple(o.npass, o.log_r[0], o.log_r[1], o.log_r[2], o.log_r[3], o.log_e, o.mfma);
     }
 };
 
@@ -169,6 +170,7 @@ static Split choose_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight) {
 
 struct PassSpec {
     uint32_t log_r = 0, log_c = 0, log_l = 0;  // rows, tile columns, product of earlier radices
+    bool mfma = false;                         // z_off then points at an MFMA table block
     int z_off = -1;
     uint32_t post = POST_NONE;
     int lo_off = -1, hi_off = -1;
@@ -224,6 +226,12 @@ struct TableBuilder {
         }
         return (int)off;
     }
+    int add_raw(const std::vector<unsigned char>& raw) {
+        size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + raw.size());
+        memcpy(&host[off], raw.data(), raw.size());
+        return (int)off;
+    }
     // Two-level table of base^e, e < 2^mbits: low 2^lob entries and high part.
     void pow_tables(PassSpec& s, uint64_t base, uint32_t mbits, uint64_t mult) {
         const uint32_t lob = (mbits + 1) / 2;
@@ -233,6 +241,72 @@ struct TableBuilder {
         s.hi_off = add(powers(powmod(base, 1ull << lob, q), 1ull << (mbits - lob), q, mult));
     }
 };
+
+// Balanced base-256 digits of v (|v| < 2^30): v = sum_i d_i 256^i, d_i in [-128, 127].
+static void balanced_digits(int64_t v, int8_t d[4]) {
+    for (int i = 0; i < 4; i++) {
+        int64_t r = ((v + 128) % 256 + 256) % 256 - 128;
+        d[i] = (int8_t)r;
+        v = (v - r) / 256;
+    }
+}
+
+static int64_t centred(uint64_t v, uint64_t q) { return v > q / 2 ? (int64_t)v - (int64_t)q : (int64_t)v; }
+
+// MFMA table block (layout in ntt_mfma.hpp) for an R-point transform with
+// root w_R, R = 64 or 4096, and negacyclic input factor `fac` = psi_{2R} (1 if cyclic).
+//   R = 64   : stage A matrix F[k][j] = w64^(jk) fac^j.
+//   R = 4096 : stage A F[k1][j1] = w64^(j1 k1) fac^(64 j1); twist w_R^(k1 j2) fac^(j2);
+//              stage B F[k2][j2] = w64^(j2 k2); j = 64 j1 + j2, k = k1 + 64 k2.
+static std::vector<unsigned char> build_mfma_tables(uint32_t log_r, uint64_t wR, uint64_t fac, uint64_t q) {
+    std::vector<unsigned char> blk(kMfmaTableBytes, 0);
+    const uint64_t w64 = powmod(wR, 1ull << (log_r - 6), q);
+    const uint64_t f64 = log_r == 12 ? powmod(fac, 64, q) : fac;
+    int8_t* frag = (int8_t*)blk.data();
+    int32_t* corr = (int32_t*)(blk.data() + kMfmaCorrOff);
+    const int nst = log_r == 12 ? 2 : 1;
+    for (int st = 0; st < nst; st++) {
+        // F[r][j]
+        std::vector<uint64_t> F(64 * 64);
+        for (uint32_t r = 0; r < 64; r++) {
+            const uint64_t wr = powmod(w64, r, q);
+            const uint64_t step = st == 0 ? mulmod(wr, f64, q) : wr;
+            uint64_t t = 1;
+            for (uint32_t j = 0; j < 64; j++) { F[r * 64 + j] = t; t = mulmod(t, step, q); }
+        }
+        for (uint32_t r = 0; r < 64; r++) {
+            uint64_t rs = 0;
+            for (uint32_t j = 0; j < 64; j++) rs = (rs + F[r * 64 + j]) % q;
+            const uint64_t c = mulmod(mulmod(128, 0x01010101ull % q, q), rs, q);
+            int8_t d[4];
+            balanced_digits(centred(c, q), d);
+            for (int i = 0; i < 4; i++) corr[(st * 64 + r) * 4 + i] = d[i];
+        }
+        for (int i = 0; i < 4; i++)
+            for (int mt = 0; mt < 4; mt++)
+                for (int ks = 0; ks < 4; ks++)
+                    for (int lane = 0; lane < 64; lane++)
+                        for (int b = 0; b < 16; b++) {
+                            const uint32_t row = 16 * mt + (lane & 15), j = 16 * ks + 4 * (lane >> 4) + (b >> 2);
+                            const uint64_t v = mulmod(F[row * 64 + j], powmod(2, 8 * (b & 3), q), q);
+                            int8_t d[4];
+                            balanced_digits(centred(v, q), d);
+                            frag[((((size_t)(st * 4 + i) * 4 + mt) * 4 + ks) * 64 + lane) * 16 + b] = d[i];
+                        }
+    }
+    if (log_r == 12) {
+        uint32_t* tw = (uint32_t*)(blk.data() + kMfmaTwistOff);
+        for (uint32_t k1 = 0; k1 < 64; k1++)
+            for (uint32_t j2 = 0; j2 < 64; j2++) {
+                const uint64_t v = mulmod(powmod(wR, (uint64_t)k1 * j2, q), powmod(fac, j2, q), q);
+                tw[(k1 * 64 + j2) * 2] = (uint32_t)v;
+                tw[(k1 * 64 + j2) * 2 + 1] = (uint32_t)shoup(v, q, 32);
+            }
+    }
+    return blk;
+}
+
+static bool mfma_ok(uint32_t log_r, Arith ar) { return ar == Arith::M32 && (log_r == 6 || log_r == 12); }
 
 // Builds the tables of one transform for a radix split N = R_0 * ... * R_{P-1}.
 // Pass i runs R_i-point transforms; L_i = R_0*...*R_{i-1} and M_i = N / L_i.
@@ -279,7 +353,16 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
         s.log_l = log_l;
         s.log_c = std::min(sp.log_e > lr ? sp.log_e - lr : 0u, log_n - lr);
         const uint64_t fac = (i == 0 && nega && !inv) ? powmod(p, N >> lr, q) : 1;
-        s.z_off = tb.add(ct_table(lr, powmod(w, N >> lr, q), fac, q));
+        // MFMA passes: R = 4096 any width; R = 64 needs >= 16 columns; never single-pass
+        s.mfma = ((sp.mfma >> i) & 1u) && mfma_ok(lr, arith) && sp.npass >= 2 && (lr == 12 || log_n - lr >= 4);
+        if (s.mfma) {
+            s.z_off = tb.add_raw(build_mfma_tables(lr, powmod(w, N >> lr, q), fac, q));
+            const uint32_t cmax = lr == 12 ? 2u : 8u;  // LDS: 4 columns of 4096, 256 of 64
+            s.log_c = std::min(s.log_c, cmax);
+            if (lr == 6) s.log_c = std::max(s.log_c, std::min(4u, log_n - lr));
+        } else {
+            s.z_off = tb.add(ct_table(lr, powmod(w, N >> lr, q), fac, q));
+        }
         if (i < P->npass - 1) {
             s.post = POST_FOURSTEP;
             if (i == 0) {
@@ -325,9 +408,15 @@ static uint64_t plan_scratch_elems(const Plan& P) { return P.npass >= 2 ? (1ull 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 // Load/store order, vector width and LDS staging flags of a job.
-static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout) {
+static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma) {
     const uint32_t R = 1u << J.log_r, C = 1u << J.log_c;
     uint32_t f = 0;
+    if (mfma) {
+        if (J.s_c == 1) f |= JF_LOAD_CFAST;
+        if (J.o_c == 1 && J.o_lbits >= 2) f |= JF_STORE_CFAST;
+        J.flags = f | JF_MFMA;
+        return;
+    }
     const bool lcf = J.s_c == 1;
     if (lcf) f |= JF_LOAD_CFAST;
     const bool scf = J.o_c == 1 && J.o_lbits >= 2;
@@ -348,6 +437,10 @@ static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout) {
 
 static uint32_t job_lds(const PassJob& J, Arith ar) {
     const uint32_t R = 1u << J.log_r;
+    if (J.flags & JF_MFMA) {
+        const uint32_t sc = J.log_r == 12 ? 64u * kMfmaRow + 8u : 72u;
+        return ((1u << J.log_c) * sc + (J.log_r == 12 ? 2u * 64u * kMfmaRow : 0u)) * 4u;
+    }
     uint32_t b = (R << J.log_c) * vbytes(ar);
     if (J.flags & JF_Z_LDS) b += R * 2 * vbytes(ar);
     return b;
@@ -405,7 +498,7 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
             J.npoly = (uint32_t)npoly;
             J.in_pstride = J.out_pstride = N;
         }
-        finish_flags(J, P.arith, src_st, dst_st);
+        finish_flags(J, P.arith, src_st, dst_st, s.mfma);
         src = dst;
         src_st = dst_st;
     }
@@ -428,10 +521,30 @@ static ntt_status launch_t(const PassJob* djobs, uint32_t njobs, const PassJob& 
     return NTT_OK;
 }
 
+template <class Sin, class Sout>
+static ntt_status launch_mfma_t(const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t ntiles, uint32_t lds,
+                                hipStream_t stream) {
+    auto fn = mfma_pass_kernel<Sin, Sout>;
+    static std::once_flag once;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    });
+    const uint32_t grid = std::min<uint32_t>(ntiles, 512);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), lds, stream, djobs, njobs, inl, ntiles);
+    HIP_TRY(hipGetLastError());
+    return NTT_OK;
+}
+
 // One launch of the pass kernel for (arith, input storage, output storage).
-static ntt_status launch_pass(Arith ar, Store sin, Store sout, const PassJob* djobs, uint32_t njobs,
+static ntt_status launch_pass(Arith ar, Store sin, Store sout, bool mfma, const PassJob* djobs, uint32_t njobs,
                               const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream) {
     if (nblocks == 0) return NTT_OK;
+    if (mfma) {
+        if (sin == Store::U32 && sout == Store::U32) return launch_mfma_t<uint32_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
+        if (sin == Store::U64 && sout == Store::U32) return launch_mfma_t<uint64_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
+        if (sin == Store::U32 && sout == Store::U64) return launch_mfma_t<uint32_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
+        return launch_mfma_t<uint64_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
+    }
     if (ar == Arith::M32) {
         if (sin == Store::U32 && sout == Store::U32) return launch_t<Mod32, uint32_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
         if (sin == Store::U64 && sout == Store::U32) return launch_t<Mod32, uint64_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
@@ -447,7 +560,8 @@ static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, 
     Store io[4][2];
     make_jobs(P, data, st, npoly, scratch, jobs, io);
     for (int i = 0; i < P.npass; i++) {
-        ntt_status s = launch_pass(P.arith, io[i][0], io[i][1], nullptr, 0, jobs[i], job_tiles(jobs[i]),
+        ntt_status s = launch_pass(P.arith, io[i][0], io[i][1], (jobs[i].flags & JF_MFMA) != 0, nullptr, 0, jobs[i],
+                                   job_tiles(jobs[i]),
                                    job_lds(jobs[i], P.arith), stream);
         if (s != NTT_OK) return s;
     }
@@ -794,6 +908,7 @@ struct ntt_c_batch {
     struct Group {
         Arith ar;
         Store sin, sout;
+        bool mfma = false;
         uint32_t njobs = 0, nblocks = 0, lds = 0;
         PassJob* djobs = nullptr;
         PassJob inl{};
@@ -855,8 +970,10 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
     // Group by (phase, arith, storage); within a group order by LDS size so
     // that big tiles start first.
     std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
-        return std::make_tuple(a.phase, (int)a.ar, (int)a.sin, (int)a.sout, -(int)(a.job.log_r + a.job.log_c)) <
-               std::make_tuple(b.phase, (int)b.ar, (int)b.sin, (int)b.sout, -(int)(b.job.log_r + b.job.log_c));
+        return std::make_tuple(a.phase, (int)a.ar, (int)a.sin, (int)a.sout, (int)(a.job.flags & JF_MFMA),
+                               -(int)(a.job.log_r + a.job.log_c)) <
+               std::make_tuple(b.phase, (int)b.ar, (int)b.sin, (int)b.sout, (int)(b.job.flags & JF_MFMA),
+                               -(int)(b.job.log_r + b.job.log_c));
     });
     for (size_t i = 0; i < items.size();) {
         size_t j = i;
@@ -865,10 +982,12 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
         memset(&g.inl, 0, sizeof(g.inl));
         g.sin = items[i].sin;
         g.sout = items[i].sout;
+        g.mfma = (items[i].job.flags & JF_MFMA) != 0;
         std::vector<PassJob> jobs;
         uint32_t blocks = 0, lds = 0;
         while (j < items.size() && items[j].phase == items[i].phase && items[j].ar == g.ar && items[j].sin == g.sin &&
-               items[j].sout == g.sout) {
+               items[j].sout == g.sout &&
+               (items[j].job.flags & JF_MFMA) == (items[i].job.flags & JF_MFMA)) {
             PassJob J = items[j].job;
             J.tile_begin = blocks;
             blocks += job_tiles(J);
@@ -894,7 +1013,7 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
 static ntt_status batch_enqueue(ntt_c_batch* B, hipStream_t stream) {
     for (auto& ph : B->phases)
         for (auto& g : ph) {
-            ntt_status st = launch_pass(g.ar, g.sin, g.sout, g.djobs, g.njobs, g.inl, g.nblocks, g.lds, stream);
+            ntt_status st = launch_pass(g.ar, g.sin, g.sout, g.mfma, g.djobs, g.njobs, g.inl, g.nblocks, g.lds, stream);
             if (st != NTT_OK) return st;
         }
     return NTT_OK;
@@ -987,12 +1106,13 @@ extern "C" ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* d, v
 }
 
 extern "C" ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc* d, uint32_t npass,
-                                            const uint32_t* log_r, uint32_t log_e, void* stream) {
+                                            const uint32_t* log_r, uint32_t log_e, uint32_t mfma_mask, void* stream) {
     if (!log_r || npass < 1 || npass > 4) return fail(NTT_ERR_INVALID_ARG, "transform_split: bad split");
     Split sp;
     sp.npass = npass;
     for (uint32_t i = 0; i < npass; i++) sp.log_r[i] = log_r[i];
     sp.log_e = log_e;
+    sp.mfma = mfma_mask & ((1u << npass) - 1u);
     if (d) {
         const uint32_t emax = max_log_e(arith_for(d->q, d->flags));
         for (uint32_t i = 0; i < npass; i++)

@@ -85,7 +85,7 @@ def lib():
         "ntt_c_context_create": [P(vp)],
         "ntt_c_context_destroy": [vp],
         "ntt_c_transform": [vp, P(Desc), vp],
-        "ntt_c_transform_split": [vp, P(Desc), u32, P(u32), u32, vp],
+        "ntt_c_transform_split": [vp, P(Desc), u32, P(u32), u32, u32, vp],
         "ntt_c_batch_prepare": [vp, P(Desc), ctypes.c_size_t, P(vp)],
         "ntt_c_batch_launch": [vp, vp],
         "ntt_c_batch_launch_graph": [vp, vp],
@@ -310,10 +310,10 @@ class Context:
         d = Desc(data_ptr, q, log_n, flags)
         check(lib().ntt_c_transform(self.ptr, ctypes.byref(d), _sp(stream)))
 
-    def transform_split(self, data_ptr, q, log_n, flags, log_r, log_e, stream=None):
+    def transform_split(self, data_ptr, q, log_n, flags, log_r, log_e, stream=None, mfma=0):
         d = Desc(data_ptr, q, log_n, flags)
         arr = (ctypes.c_uint32 * len(log_r))(*log_r)
-        check(lib().ntt_c_transform_split(self.ptr, ctypes.byref(d), len(log_r), arr, log_e, _sp(stream)))
+        check(lib().ntt_c_transform_split(self.ptr, ctypes.byref(d), len(log_r), arr, log_e, mfma, _sp(stream)))
 
     def destroy(self):
         if self.ptr:

@@ -57,3 +57,58 @@ def test_split_rejects_bad_shapes(nt, ctx):
         ctx.transform_split(buf.ptr, BB, 10, 0, [4, 4], 12)      # does not cover n
     with pytest.raises(nt.NttError):
         ctx.transform_split(buf.ptr, BB, 10, 0, [10], 15)        # tile too large
+
+
+MFMA_SPLITS = [
+    (18, [12, 6], 14, 1), (18, [12, 6], 14, 3), (18, [6, 12], 14, 2), (16, [6, 10], 12, 1),
+    (24, [12, 12], 14, 3), (20, [12, 8], 14, 1), (12, [6, 6], 10, 3), (22, [6, 6, 10], 12, 3),
+]
+
+
+@pytest.mark.parametrize("log_n,split,log_e,mask", MFMA_SPLITS)
+@pytest.mark.parametrize("kind", ["cyc", "icyc", "nega", "inega"])
+@pytest.mark.parametrize("u64", [False, True])
+def test_mfma_split(nt, orc, ctx, log_n, split, log_e, mask, kind, u64):
+    """INT8-MFMA passes (R = 64 and 4096) against the oracle, bit-exact; at
+    n = 2^24 through size-independent properties (round trip, sampled outputs)."""
+    if log_n >= 22 and (u64 or kind in ("icyc", "inega")):
+        pytest.skip("one direction per kind at the largest sizes")
+    q = BB
+    n = 1 << log_n
+    flags = {"cyc": 0, "icyc": nt.FLAG_INVERSE, "nega": nt.FLAG_NEGACYCLIC,
+             "inega": nt.FLAG_NEGACYCLIC | nt.FLAG_INVERSE}[kind] | (nt.FLAG_U64 if u64 else 0)
+    dt = np.uint64 if u64 else np.uint32
+    x = orc.random_poly(n, q, 11 * log_n + mask)
+    buf = nt.DeviceBuffer.from_array(x.astype(dt))
+    ctx.transform_split(buf.ptr, q, log_n, flags, split, log_e, mfma=mask)
+    nt.device_sync()
+    got = buf.download(dt, n).astype(np.uint64)
+    if log_n <= 20:
+        assert np.array_equal(got, orc.transform(x, q, "nega" in kind, kind.startswith("i"))), (split, mask)
+    else:
+        # sampled outputs from the defining sums, then the inverse through the VALU path
+        g = orc.generator(q)
+        if kind == "nega":
+            psi = pow(g, (q - 1) // (2 * n), q)
+            for k in (0, 3, n // 3, n - 1):
+                p = _pw(pow(psi, 2 * k + 1, q), n, q)
+                assert int(got[k]) == int(((x * p) % np.uint64(q)).sum(dtype=np.uint64) % np.uint64(q)), k
+        else:
+            w = pow(g, (q - 1) // n, q)
+            for k in (0, 5, n // 7, n - 1):
+                p = _pw(pow(w, k, q), n, q)
+                assert int(got[k]) == int(((x * p) % np.uint64(q)).sum(dtype=np.uint64) % np.uint64(q)), k
+        ctx.transform(buf.ptr, q, log_n, flags ^ nt.FLAG_INVERSE)
+        nt.device_sync()
+        assert np.array_equal(buf.download(dt, n).astype(np.uint64), x)
+
+
+def _pw(r, n, q):
+    p = np.ones(n, dtype=np.uint64)
+    m, rm = 1, r % q
+    while m < n:
+        k = min(m, n - m)
+        p[m:m + k] = (p[:k] * np.uint64(rm)) % np.uint64(q)
+        rm = rm * rm % q
+        m *= 2
+    return p
