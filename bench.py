@@ -234,6 +234,22 @@ def extra_serving(ctx, qps_list=(100, 500, 1000, 2000, 4000, 8000), count=120):
     return out
 
 
+def tune_all(ctx, args):
+    """Autotune every launch class the bench measures (choices keyed by size and
+    elements in flight); returns the chosen splits next to the heuristic."""
+    out = {}
+    def rec(name, q, log_ns, elems):
+        for r in nt.autotune(ctx, q, log_ns, elems, reps=4):
+            out[f"{name}:2^{r['log_n']}"] = {k: r[k] for k in ("split", "log_e", "mfma_mask", "us", "heuristic_us")}
+    rec("single", BB, [args.log_n, 20], 1)
+    rec("batch_2^24", BB, [20, 16], 1 << 24)
+    rec("rns", BB, [16], 24 << 16)
+    rec("uneven", BB, list(range(10, 19)), 1 << 24)
+    rec("serve_bb", BB, list(range(12, 23)), 1 << 24)
+    rec("serve_62", W.p62_primes()[0], list(range(12, 23)), 1 << 24)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +258,7 @@ def main():
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the NTT path runs on a single MI355X "
@@ -252,6 +269,10 @@ def main():
         sys.exit(1)
     stream = nt.Stream()
     ctx = nt.Context()
+    tuned = {}
+    if not args.no_tune:
+        log("[bench] autotune")
+        tuned = tune_all(ctx, args)
     log(f"[bench] headline: negacyclic n=2^{args.log_n} fwd+inv, {args.warmup} warmup + {args.steps} steps")
     h = headline(args, ctx, stream)
     log(f"[bench] headline done: {h['value']:.1f} Gbfly/s, {h['roofline']['achieved']:.0f} GB/s")
@@ -277,7 +298,7 @@ def main():
         "config": {"workload": f"negacyclic NTT over BabyBear q=2013265921, n=2^{args.log_n}, forward then "
                                "inverse, one polynomial, uint32 coefficients resident in HBM",
                    "log_n": args.log_n, "q": BB, "transform": "negacyclic", "step": "forward+inverse"},
-        "roofline": h["roofline"], "cpu_baseline": cpu, "extras": extras,
+        "roofline": h["roofline"], "cpu_baseline": cpu, "extras": extras, "autotune": tuned,
     }
     print(json.dumps(out), flush=True)
     ctx.destroy()

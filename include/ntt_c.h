@@ -171,6 +171,22 @@ ntt_status ntt_c_batch_destroy(ntt_c_batch* batch);
 ntt_status ntt_c_batch_info(const ntt_c_batch* batch, uint32_t* launches, uint64_t* workspace_bytes,
                             uint64_t* workgroups);
 
+/* ---------------- autotuner (no reference counterpart; the reference's split
+ * is fixed at radix-64 rounds, src/ntt_optimized.cu:529-554) ----------------
+ * For each log_n, times candidate radix splits / tile sizes / VALU-or-MFMA
+ * passes on a uniform batch of elems_in_flight elements (modulus q) and
+ * installs the fastest for that (size, launch-size class); later plans use it. */
+typedef struct {
+    uint32_t log_n, npass, log_r[4], log_e, mfma_mask, candidates;
+    double us_per_transform;  /* tuned choice */
+    double heuristic_us;      /* the built-in heuristic on the same batch */
+} ntt_c_tune_result;
+ntt_status ntt_c_autotune(ntt_c_context* ctx, uint64_t q, const uint32_t* log_ns, size_t count,
+                          uint64_t elems_in_flight, uint32_t reps, ntt_c_tune_result* out);
+ntt_status ntt_c_autotune_save(const char* path);
+ntt_status ntt_c_autotune_load(const char* path);
+ntt_status ntt_c_autotune_clear(void);
+
 /* ---------------- continuous-batching server (no reference counterpart) ----------------
  * Requests are lists of device-resident polynomials (ntt_c_desc, transformed in
  * place).  A scheduler thread admits queued requests in arrival order while

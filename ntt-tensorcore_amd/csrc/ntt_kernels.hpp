@@ -191,7 +191,7 @@ __device__ __forceinline__ void store4<uint64_t>(uint64_t* p, const uint64_t v[4
 }
 
 // Loads per thread kept in flight before their LDS stores.
-constexpr int kUnroll = 4;
+constexpr int kUnroll = 8;
 
 template <class M, class Sin, class Sout, int MAXM>
 __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job, unsigned char* lds_raw) {

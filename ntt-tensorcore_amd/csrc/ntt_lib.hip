@@ -412,8 +412,18 @@ static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma)
     const uint32_t R = 1u << J.log_r, C = 1u << J.log_c;
     uint32_t f = 0;
     if (mfma) {
-        if (J.s_c == 1) f |= JF_LOAD_CFAST;
-        if (J.o_c == 1 && J.o_lbits >= 2) f |= JF_STORE_CFAST;
+        const int64_t ib = (int64_t)sbytes(sin), ob = (int64_t)sbytes(sout);
+        if (J.s_c == 1) {
+            f |= JF_LOAD_CFAST;
+            if (aligned16(J.in) && (J.in_pstride * ib) % 16 == 0 && C >= 4 && J.s_r % 4 == 0 && J.ncols % 4 == 0)
+                f |= JF_VEC4;
+        }
+        const bool scf = J.o_c == 1 && J.o_lbits >= 2;
+        if (scf) f |= JF_STORE_CFAST;
+        bool vs = aligned16(J.out) && (J.out_pstride * ob) % 16 == 0 && (J.o_hs % 4) == 0;
+        if (scf) vs = vs && C >= 4 && (J.o_r % 4) == 0 && (J.ncols % 4) == 0;
+        else vs = vs && J.o_r == 1 && (J.o_c % 4 == 0 || J.o_lbits == 0);
+        if (vs) f |= JF_VEC4_ST;
         J.flags = f | JF_MFMA;
         return;
     }
@@ -439,7 +449,7 @@ static uint32_t job_lds(const PassJob& J, Arith ar) {
     const uint32_t R = 1u << J.log_r;
     if (J.flags & JF_MFMA) {
         const uint32_t sc = J.log_r == 12 ? 64u * kMfmaRow + 8u : 72u;
-        return ((1u << J.log_c) * sc + (J.log_r == 12 ? 2u * 64u * kMfmaRow : 0u)) * 4u;
+        return ((1u << J.log_c) * sc + (J.log_r == 12 ? 2u * 64u * kMfmaRow + 2u * 4096u : 0u)) * 4u;
     }
     uint32_t b = (R << J.log_c) * vbytes(ar);
     if (J.flags & JF_Z_LDS) b += R * 2 * vbytes(ar);
@@ -1123,6 +1133,7 @@ extern "C" ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc
 }
 
 #include "ntt_serve.inc"
+#include "ntt_tune.inc"
 
 // ------------------------------------------------------------------ device helpers
 extern "C" ntt_status ntt_c_device_count(int* c) {

@@ -30,6 +30,7 @@ constexpr uint32_t kMfmaCorrOff = kMfmaFragBytes;                    // int32 [s
 constexpr uint32_t kMfmaTwistOff = kMfmaCorrOff + 2u * 64u * 4u * 4u;  // TwP<uint32_t>[k1 * 64 + j2]
 constexpr uint32_t kMfmaTableBytes = kMfmaTwistOff + 4096u * 8u;
 constexpr uint32_t kMfmaRow = 68;  // LDS words per 64-element row (pad keeps b128 reads conflict-free)
+constexpr int kMfmaUnroll = 8;     // 16-byte global accesses in flight per thread
 
 __device__ __forceinline__ uint32_t mf_col_words(uint32_t log_r) { return log_r == 12 ? 64u * kMfmaRow + 8u : 72u; }
 
@@ -68,26 +69,56 @@ __device__ __forceinline__ void mfma_tile(const PassJob& J, uint32_t tile_in_job
     auto xaddr = [&](uint32_t r, uint32_t c) -> uint32_t {
         return c * SC + (r & ((1u << lg) - 1u)) * kMfmaRow + (r >> lg);
     };
-    for (uint32_t i0 = tid; i0 < total; i0 += kUnroll * T) {
-        Sin v[kUnroll];
-        uint32_t la[kUnroll];
+    if (flags & JF_VEC4) {
+        // 16-byte row segments, kMfmaUnroll in flight per thread.  With one
+        // segment per row (C == 4) consecutive lanes take rows 64 apart, so
+        // their LDS words (consecutive j1) fall in distinct banks.
+        const uint32_t qpr = C >> 2;  // quads per row
+        const uint32_t nq = total >> 2;
+        for (uint32_t q0 = tid; q0 < nq; q0 += kMfmaUnroll * T) {
+            Sin v[kMfmaUnroll][4];
+            uint32_t rr[kMfmaUnroll], cc[kMfmaUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const uint32_t idx = i0 + u * T;
-            uint32_t r, c;
-            if (flags & JF_LOAD_CFAST) { r = idx >> log_c; c = idx & (C - 1u); }
-            else { c = idx >> log_r; r = idx & (R - 1u); }
-            la[u] = xaddr(r & (R - 1u), c & (C - 1u));
-            v[u] = 0;
-            if (idx < total && col0 + c < ncols) v[u] = in[(int64_t)r * J.s_r + (int64_t)(col0 + c) * J.s_c];
+            for (int u = 0; u < kMfmaUnroll; u++) {
+                const uint32_t qi = q0 + u * T;
+                uint32_t r = qi / qpr;
+                cc[u] = (qi - r * qpr) << 2;
+                if (two && qpr == 1) r = ((r & 63u) << 6) | (r >> 6);
+                rr[u] = r;
+                if (qi < nq && col0 + cc[u] < ncols) load4<Sin>(in + (int64_t)r * J.s_r + (int64_t)(col0 + cc[u]), v[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kMfmaUnroll; u++) {
+                const uint32_t qi = q0 + u * T;
+                if (qi >= nq) continue;
+                const bool ok = col0 + cc[u] < ncols;
+#pragma unroll
+                for (int e = 0; e < 4; e++) lds[xaddr(rr[u], cc[u] + e)] = ok ? (uint32_t)v[u][e] : 0u;
+            }
         }
+    } else {
+        for (uint32_t i0 = tid; i0 < total; i0 += kUnroll * T) {
+            Sin v[kUnroll];
+            uint32_t la[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++)
-            if (i0 + u * T < total) lds[la[u]] = (uint32_t)v[u];
+            for (int u = 0; u < kUnroll; u++) {
+                const uint32_t idx = i0 + u * T;
+                uint32_t r, c;
+                if (flags & JF_LOAD_CFAST) { r = idx >> log_c; c = idx & (C - 1u); }
+                else { c = idx >> log_r; r = idx & (R - 1u); }
+                la[u] = xaddr(r & (R - 1u), c & (C - 1u));
+                v[u] = 0;
+                if (idx < total && col0 + c < ncols) v[u] = in[(int64_t)r * J.s_r + (int64_t)(col0 + c) * J.s_c];
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+                if (i0 + u * T < total) lds[la[u]] = (uint32_t)v[u];
+        }
     }
     __syncthreads();
 
-    const TwP<uint32_t>* tw = reinterpret_cast<const TwP<uint32_t>*>(reinterpret_cast<const char*>(J.z) + kMfmaTwistOff);
+    // twist table staged in LDS behind the two stage buffers (copied by the kernel prologue)
+    const TwP<uint32_t>* tw = reinterpret_cast<const TwP<uint32_t>*>(lds + C * SC + 2u * 64u * kMfmaRow);
     if (!two) {
         // R = 64: 16-column tiles; every wave reads its tiles, then all write.
         constexpr int kMaxNt = 8;  // C <= 256 columns
@@ -194,35 +225,68 @@ __device__ __forceinline__ void mfma_tile(const PassJob& J, uint32_t tile_in_job
     const TwP<uint32_t> scale{(uint32_t)J.scale_w, (uint32_t)J.scale_wp};
     const uint32_t lomask = (1u << J.lo_bits) - 1u;
     const uint32_t omask = (1u << J.o_lbits) - 1u;
-    for (uint32_t i0 = tid; i0 < total; i0 += kUnroll * T) {
-        uint32_t v[kUnroll];
-        int64_t a[kUnroll];
-        bool ok[kUnroll];
+    auto postop = [&](uint32_t x, uint32_t k, uint32_t cg) -> uint32_t {
+        if (post_mode == POST_FOURSTEP) {
+            const uint32_t uu = cg >> J.e_shift;
+            const uint32_t e = (J.e_a * k * uu + J.e_b * k + J.e_d * uu) & J.e_mask;
+            return mod.mul(mod.mul(x, tlo[e & lomask]), thi[e >> J.lo_bits]);
+        }
+        if (post_mode == POST_SCALE) return mod.mul(x, scale);
+        return x;
+    };
+    auto oaddr = [&](uint32_t k, uint32_t cg) -> int64_t {
+        return (int64_t)(cg >> J.o_lbits) * J.o_hs + (int64_t)(cg & omask) * J.o_c + (int64_t)k * J.o_r;
+    };
+    auto zaddr = [&](uint32_t k, uint32_t c) -> uint32_t { return c * SC + (k >> 6) * kMfmaRow + (k & 63u); };
+    if (flags & JF_VEC4_ST) {
+        const uint32_t nq = total >> 2;
+        for (uint32_t q0 = tid; q0 < nq; q0 += kMfmaUnroll * T) {
+            Sout v[kMfmaUnroll][4];
+            int64_t a[kMfmaUnroll];
+            bool ok[kMfmaUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const uint32_t idx = i0 + u * T;
-            uint32_t k, c;
-            if (flags & JF_STORE_CFAST) { k = idx >> log_c; c = idx & (C - 1u); }
-            else { c = idx >> log_r; k = idx & (R - 1u); }
-            ok[u] = idx < total && col0 + c < ncols;
-            const uint32_t cg = col0 + c;
-            a[u] = (int64_t)(cg >> J.o_lbits) * J.o_hs + (int64_t)(cg & omask) * J.o_c + (int64_t)k * J.o_r;
-            uint32_t x = 0;
-            if (ok[u]) {
-                x = lds[c * SC + (k >> 6) * kMfmaRow + (k & 63u)];
-                if (post_mode == POST_FOURSTEP) {
-                    const uint32_t uu = cg >> J.e_shift;
-                    const uint32_t e = (J.e_a * k * uu + J.e_b * k + J.e_d * uu) & J.e_mask;
-                    x = mod.mul(mod.mul(x, tlo[e & lomask]), thi[e >> J.lo_bits]);
-                } else if (post_mode == POST_SCALE) {
-                    x = mod.mul(x, scale);
+            for (int u = 0; u < kMfmaUnroll; u++) {
+                const uint32_t qi = q0 + u * T;
+                const uint32_t idx = qi << 2;
+                uint32_t k, c;
+                if (flags & JF_STORE_CFAST) { k = idx >> log_c; c = idx & (C - 1u); }
+                else { c = idx >> log_r; k = idx & (R - 1u); }
+                ok[u] = qi < nq && col0 + c < ncols;
+                a[u] = oaddr(k, col0 + c);
+                if (ok[u]) {
+                    if (flags & JF_STORE_CFAST) {
+#pragma unroll
+                        for (int e = 0; e < 4; e++) v[u][e] = (Sout)postop(lds[zaddr(k, c + e)], k, col0 + c + e);
+                    } else {
+                        const v4i z = *reinterpret_cast<const v4i*>(lds + zaddr(k, c));
+#pragma unroll
+                        for (int e = 0; e < 4; e++) v[u][e] = (Sout)postop((uint32_t)z[e], k + e, col0 + c);
+                    }
                 }
             }
-            v[u] = x;
-        }
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++)
-            if (ok[u]) out[a[u]] = (Sout)v[u];
+            for (int u = 0; u < kMfmaUnroll; u++)
+                if (ok[u]) store4<Sout>(out + a[u], v[u]);
+        }
+    } else {
+        for (uint32_t i0 = tid; i0 < total; i0 += kUnroll * T) {
+            uint32_t v[kUnroll];
+            int64_t a[kUnroll];
+            bool ok[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const uint32_t idx = i0 + u * T;
+                uint32_t k, c;
+                if (flags & JF_STORE_CFAST) { k = idx >> log_c; c = idx & (C - 1u); }
+                else { c = idx >> log_r; k = idx & (R - 1u); }
+                ok[u] = idx < total && col0 + c < ncols;
+                a[u] = oaddr(k, col0 + c);
+                v[u] = ok[u] ? postop(lds[zaddr(k, c)], k, col0 + c) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+                if (ok[u]) out[a[u]] = (Sout)v[u];
+        }
     }
     __syncthreads();
 }
@@ -237,6 +301,7 @@ __global__ void __launch_bounds__(512) mfma_pass_kernel(const PassJob* __restric
     const uint32_t lane = threadIdx.x & 63u, mt = (threadIdx.x >> 6) & 3u, g = lane >> 4;
     v4i fa[4][4], cav[4];
     const void* cur = nullptr;
+    const void* tw_cur = nullptr;
     for (uint32_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
         uint32_t j = 0;
         if (njobs) {
@@ -261,6 +326,16 @@ __global__ void __launch_bounds__(512) mfma_pass_kernel(const PassJob* __restric
 #pragma unroll
                 for (int e = 0; e < 4; e++) cav[i][e] = cr[(stg * 64 + 16 * mt + 4 * g + e) * 4 + i];
             }
+        }
+        if (J.log_r == 12 && J.z != tw_cur) {
+            // stage the 4096-entry twist table once per plan (region after the tile and stage buffers)
+            __syncthreads();
+            tw_cur = J.z;
+            const uint32_t off = ((1u << J.log_c) * mf_col_words(12) + 2u * 64u * kMfmaRow);
+            const uint2* src = reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(J.z) + kMfmaTwistOff);
+            uint2* dst = reinterpret_cast<uint2*>(lds + off);
+            for (uint32_t i = threadIdx.x; i < 4096u; i += blockDim.x) dst[i] = src[i];
+            __syncthreads();
         }
         mfma_tile<Sin, Sout>(J, t - J.tile_begin, lds, fa, cav);
     }

@@ -425,3 +425,48 @@ class Server:
         if self.ptr:
             check(lib().ntt_c_server_destroy(self.ptr))
             self.ptr = None
+
+
+# ---------------------------------------------------------------- autotuner
+class TuneResult(ctypes.Structure):
+    _fields_ = [("log_n", ctypes.c_uint32), ("npass", ctypes.c_uint32), ("log_r", ctypes.c_uint32 * 4),
+                ("log_e", ctypes.c_uint32), ("mfma_mask", ctypes.c_uint32), ("candidates", ctypes.c_uint32),
+                ("us_per_transform", ctypes.c_double), ("heuristic_us", ctypes.c_double)]
+
+
+def _tune_sigs(L):
+    P = ctypes.POINTER
+    L.ntt_c_autotune.argtypes = [ctypes.c_void_p, ctypes.c_uint64, P(ctypes.c_uint32), ctypes.c_size_t,
+                                 ctypes.c_uint64, ctypes.c_uint32, P(TuneResult)]
+    L.ntt_c_autotune.restype = ctypes.c_int
+    for n in ("ntt_c_autotune_save", "ntt_c_autotune_load"):
+        getattr(L, n).argtypes = [ctypes.c_char_p]
+        getattr(L, n).restype = ctypes.c_int
+    L.ntt_c_autotune_clear.argtypes = []
+    L.ntt_c_autotune_clear.restype = ctypes.c_int
+
+
+_EXTRA_SIGS.append(_tune_sigs)
+
+
+def autotune(ctx, q, log_ns, elems_in_flight, reps=5):
+    arr = (ctypes.c_uint32 * len(log_ns))(*log_ns)
+    out = (TuneResult * len(log_ns))()
+    check(lib().ntt_c_autotune(ctx.ptr, q, arr, len(log_ns), elems_in_flight, reps, out))
+    res = []
+    for r in out:
+        res.append({"log_n": r.log_n, "split": list(r.log_r)[:r.npass], "log_e": r.log_e, "mfma_mask": r.mfma_mask,
+                    "candidates": r.candidates, "us": r.us_per_transform, "heuristic_us": r.heuristic_us})
+    return res
+
+
+def autotune_save(path):
+    check(lib().ntt_c_autotune_save(path.encode()))
+
+
+def autotune_load(path):
+    check(lib().ntt_c_autotune_load(path.encode()))
+
+
+def autotune_clear():
+    check(lib().ntt_c_autotune_clear())

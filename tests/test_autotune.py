@@ -1,0 +1,44 @@
+"""Autotuner: the saved table round-trips through load/save (CPU); on the
+GPU it picks a split at least as fast as the heuristic and transforms made
+with the tuned plans stay bit-exact."""
+import os
+
+import numpy as np
+import pytest
+
+BB = 2013265921
+
+
+def test_tune_table_load_save_roundtrip(nt, tmp_path):
+    src = tmp_path / "t.txt"
+    src.write_text("0 24 24 2 12 12 0 0 14 3\n0 20 20 3 7 7 6 0 12 0\n1 18 18 2 9 9 0 0 13 0\n"
+                   "0 20 20 2 9 9 0 0 12 0\n")  # last line does not cover n: ignored
+    nt.autotune_clear()
+    nt.autotune_load(str(src))
+    dst = tmp_path / "u.txt"
+    nt.autotune_save(str(dst))
+    lines = sorted(dst.read_text().split("\n"))
+    assert [l for l in lines if l] == sorted(["0 24 24 2 12 12 0 0 14 3", "0 20 20 3 7 7 6 0 12 0",
+                                             "1 18 18 2 9 9 0 0 13 0"])
+    nt.autotune_clear()
+    with pytest.raises(nt.NttError):
+        nt.autotune_load(str(tmp_path / "missing.txt"))
+
+
+@pytest.mark.gpu
+def test_autotune_then_transform_exact(nt, orc):
+    ctx = nt.Context()
+    res = nt.autotune(ctx, BB, [12, 16, 20], 1 << 20, reps=3)
+    for r in res:
+        assert r["candidates"] >= 3 and sum(r["split"]) == r["log_n"]
+        assert r["us"] <= r["heuristic_us"] * 1.05 + 1.0
+    for log_n in (16, 20):
+        x = orc.random_poly(1 << log_n, BB, 5)
+        buf = nt.DeviceBuffer.from_array(x.astype(np.uint32))
+        b = nt.Batch(ctx, [(buf.ptr, BB, log_n, nt.FLAG_NEGACYCLIC)] * 1)
+        b.launch()
+        nt.device_sync()
+        assert np.array_equal(buf.download(np.uint32, 1 << log_n).astype(np.uint64), orc.ntt_negacyclic(x, BB))
+        b.destroy()
+    nt.autotune_clear()
+    ctx.destroy()
