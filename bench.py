@@ -58,9 +58,18 @@ def headline(args, ctx, stream):
     fwd = nt.FLAG_NEGACYCLIC
     inv = nt.FLAG_NEGACYCLIC | nt.FLAG_INVERSE
 
-    def step():
-        ctx.transform(buf.ptr, BB, log_n, fwd, stream)
-        ctx.transform(buf.ptr, BB, log_n, inv, stream)
+    if args.force_split:
+        split = [int(v) for v in args.force_split.split(",")]
+
+        def step():
+            ctx.transform_split(buf.ptr, BB, log_n, fwd, split, args.force_log_e, stream, mfma=args.force_mfma)
+            ctx.transform_split(buf.ptr, BB, log_n, inv, split, args.force_log_e, stream, mfma=args.force_mfma)
+        plan = {"split": split, "log_e": args.force_log_e, "mfma_mask": args.force_mfma, "forced": True}
+    else:
+        def step():
+            ctx.transform(buf.ptr, BB, log_n, fwd, stream)
+            ctx.transform(buf.ptr, BB, log_n, inv, stream)
+        plan = nt.split_for(BB, log_n, n)
 
     ms = timed(step, args.steps, args.warmup, stream)
     stream.sync()
@@ -72,11 +81,19 @@ def headline(args, ctx, stream):
     bfly = 2 * W.butterflies(log_n)
     t_transform_s = ms_step / 2 / 1e3
     achieved = 2 * n * 4 / t_transform_s / 1e9
+    traffic = None
+    tf = os.path.join(REPO, "profiles", f"pass_kernel_negacyclic_bb_2e{log_n}_traffic.json")
+    if os.path.exists(tf):
+        t = json.load(open(tf))
+        if t.get("log_n") == log_n and t.get("split") == plan["split"] and t.get("log_e") == plan["log_e"] \
+                and t.get("mfma_mask") == plan["mfma_mask"]:
+            traffic = t["hbm_bytes_per_transform"]
     return {
+        "plan": plan,
         "ms_per_step": ms_step,
         "value": bfly * args.steps / (ms / 1e3) / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "per": "one transform (2*n*4 B, both pass-kernel launches)",
                      "transform_us": t_transform_s * 1e6},
     }
@@ -258,7 +275,11 @@ def main():
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--retune", action="store_true", help="re-run the autotuner instead of loading its table")
+    ap.add_argument("--force-split", default="", help="headline radix split, e.g. 12,12 (profiling other plans)")
+    ap.add_argument("--force-log-e", type=int, default=14)
+    ap.add_argument("--force-mfma", type=int, default=0, help="mask of passes on the matrix cores")
+    ap.add_argument("--tune-out", default="", help="write the tuned table here")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the NTT path runs on a single MI355X "
@@ -270,9 +291,15 @@ def main():
     stream = nt.Stream()
     ctx = nt.Context()
     tuned = {}
-    if not args.no_tune:
+    table = os.path.join(REPO, "ntt-tensorcore_amd", "tuned_mi355x.txt")
+    if args.retune or not os.path.exists(table):
         log("[bench] autotune")
         tuned = tune_all(ctx, args)
+        if args.tune_out:
+            nt.autotune_save(args.tune_out)
+    else:
+        nt.autotune_load(table)
+        tuned = {"loaded": os.path.relpath(table, REPO)}
     log(f"[bench] headline: negacyclic n=2^{args.log_n} fwd+inv, {args.warmup} warmup + {args.steps} steps")
     h = headline(args, ctx, stream)
     log(f"[bench] headline done: {h['value']:.1f} Gbfly/s, {h['roofline']['achieved']:.0f} GB/s")
@@ -298,7 +325,7 @@ def main():
         "config": {"workload": f"negacyclic NTT over BabyBear q=2013265921, n=2^{args.log_n}, forward then "
                                "inverse, one polynomial, uint32 coefficients resident in HBM",
                    "log_n": args.log_n, "q": BB, "transform": "negacyclic", "step": "forward+inverse"},
-        "roofline": h["roofline"], "cpu_baseline": cpu, "extras": extras, "autotune": tuned,
+        "roofline": h["roofline"], "plan": h["plan"], "cpu_baseline": cpu, "extras": extras, "autotune": tuned,
     }
     print(json.dumps(out), flush=True)
     ctx.destroy()

@@ -186,6 +186,9 @@ ntt_status ntt_c_autotune(ntt_c_context* ctx, uint64_t q, const uint32_t* log_ns
 ntt_status ntt_c_autotune_save(const char* path);
 ntt_status ntt_c_autotune_load(const char* path);
 ntt_status ntt_c_autotune_clear(void);
+/* The split now chosen for 2^log_n with elems_in_flight elements per launch:
+ * out = {npass, log_r[0..3], log_e, mfma_mask}. */
+ntt_status ntt_c_split_for(uint64_t q, uint32_t log_n, uint64_t elems_in_flight, uint32_t out[7]);
 
 /* ---------------- continuous-batching server (no reference counterpart) ----------------
  * Requests are lists of device-resident polynomials (ntt_c_desc, transformed in

@@ -444,6 +444,8 @@ def _tune_sigs(L):
         getattr(L, n).restype = ctypes.c_int
     L.ntt_c_autotune_clear.argtypes = []
     L.ntt_c_autotune_clear.restype = ctypes.c_int
+    L.ntt_c_split_for.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, P(ctypes.c_uint32)]
+    L.ntt_c_split_for.restype = ctypes.c_int
 
 
 _EXTRA_SIGS.append(_tune_sigs)
@@ -470,3 +472,9 @@ def autotune_load(path):
 
 def autotune_clear():
     check(lib().ntt_c_autotune_clear())
+
+
+def split_for(q, log_n, elems_in_flight):
+    o = (ctypes.c_uint32 * 7)()
+    check(lib().ntt_c_split_for(q, log_n, elems_in_flight, o))
+    return {"split": list(o)[1:1 + o[0]], "log_e": o[5], "mfma_mask": o[6]}
