@@ -357,11 +357,12 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
         s.mfma = ((sp.mfma >> i) & 1u) && mfma_ok(lr, arith) && sp.npass >= 2 && (lr == 12 || log_n - lr >= 4);
         if (s.mfma) {
             s.z_off = tb.add_raw(build_mfma_tables(lr, powmod(w, N >> lr, q), fac, q));
-            const uint32_t cmax = lr == 12 ? 2u : 8u;  // LDS: 4 columns of 4096, 256 of 64
+            const uint32_t cmax = lr == 12 ? 3u : 8u;  // LDS: 8 columns of 4096, 256 of 64
             s.log_c = std::min(s.log_c, cmax);
             if (lr == 6) s.log_c = std::max(s.log_c, std::min(4u, log_n - lr));
         } else {
             s.z_off = tb.add(ct_table(lr, powmod(w, N >> lr, q), fac, q));
+            s.log_c = std::min(s.log_c, max_log_e(arith) > lr ? max_log_e(arith) - lr : 0u);  // VALU tile fits LDS
         }
         if (i < P->npass - 1) {
             s.post = POST_FOURSTEP;
@@ -449,7 +450,7 @@ static uint32_t job_lds(const PassJob& J, Arith ar) {
     const uint32_t R = 1u << J.log_r;
     if (J.flags & JF_MFMA) {
         const uint32_t sc = J.log_r == 12 ? 64u * kMfmaRow + 8u : 72u;
-        return ((1u << J.log_c) * sc + (J.log_r == 12 ? 2u * 64u * kMfmaRow + 2u * 4096u : 0u)) * 4u;
+        return (1u << J.log_c) * sc * 4u;
     }
     uint32_t b = (R << J.log_c) * vbytes(ar);
     if (J.flags & JF_Z_LDS) b += R * 2 * vbytes(ar);
@@ -539,7 +540,14 @@ static ntt_status launch_mfma_t(const PassJob* djobs, uint32_t njobs, const Pass
     std::call_once(once, [&] {
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     });
-    const uint32_t grid = std::min<uint32_t>(ntiles, 512);
+    // persistent: one resident workgroup per CU (LDS and registers allow one)
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cus);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), lds, stream, djobs, njobs, inl, ntiles);
     HIP_TRY(hipGetLastError());
     return NTT_OK;
@@ -1127,7 +1135,7 @@ extern "C" ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc
         const uint32_t emax = max_log_e(arith_for(d->q, d->flags));
         for (uint32_t i = 0; i < npass; i++)
             if (log_r[i] < 1 || log_r[i] > emax) return fail(NTT_ERR_INVALID_ARG, "transform_split: radix too large");
-        if (log_e > emax) return fail(NTT_ERR_INVALID_ARG, "transform_split: tile too large");
+        if (log_e > emax + 1) return fail(NTT_ERR_INVALID_ARG, "transform_split: tile too large");
     }
     return transform_one(ctx, d, &sp, stream);
 }

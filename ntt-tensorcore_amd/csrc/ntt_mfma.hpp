@@ -117,8 +117,7 @@ __device__ __forceinline__ void mfma_tile(const PassJob& J, uint32_t tile_in_job
     }
     __syncthreads();
 
-    // twist table staged in LDS behind the two stage buffers (copied by the kernel prologue)
-    const TwP<uint32_t>* tw = reinterpret_cast<const TwP<uint32_t>*>(lds + C * SC + 2u * 64u * kMfmaRow);
+    const TwP<uint32_t>* tw = reinterpret_cast<const TwP<uint32_t>*>(reinterpret_cast<const char*>(J.z) + kMfmaTwistOff);
     if (!two) {
         // R = 64: 16-column tiles; every wave reads its tiles, then all write.
         constexpr int kMaxNt = 8;  // C <= 256 columns
@@ -155,21 +154,30 @@ __device__ __forceinline__ void mfma_tile(const PassJob& J, uint32_t tile_in_job
         }
         __syncthreads();
     } else {
-        // Column pipeline: at step s waves 0-3 run stage A of column s into
-        // ys[s & 1] while waves 4-7 run stage B of column s-1 from ys[(s-1) & 1].
-        uint32_t* ysb = lds + C * SC;  // two buffers of [k1][j2], 64 x 68 words each
+        // Column pipeline, in place: at step s waves 0-3 run stage A of column s
+        // (X[s] -> Y[s]) while waves 4-7 run stage B of column s-1 (Y -> Z); each
+        // half reads, holds its results in registers over a barrier, then
+        // writes them into the same column region.
         const bool stage_a = wave < 4;
         for (uint32_t st = 0; st <= C; st++) {
-            if (stage_a && st < C) {
-                const uint32_t c = st;
-                uint32_t* ys = ysb + (st & 1u) * 64u * kMfmaRow;
+            const bool active = stage_a ? st < C : st >= 1;
+            const uint32_t c = stage_a ? st : st - 1;
+            uint32_t res[4][4];
+            if (active) {
+                uint32_t* col = lds + c * SC;
+#pragma unroll
                 for (uint32_t nt = 0; nt < 4; nt++) {
-                    const uint32_t j2 = 16u * nt + n;
+                    const uint32_t u = 16u * nt + n;  // j2 (stage A) or k1 (stage B)
+                    TwP<uint32_t> tv[4];
+                    if (stage_a) {
+#pragma unroll
+                        for (int e = 0; e < 4; e++) tv[e] = tw[(16u * mt + 4u * g + e) * 64u + u];
+                    }
                     v4i b[4];
 #pragma unroll
                     for (int ks = 0; ks < 4; ks++) {
-                        b[ks] = *reinterpret_cast<const v4i*>(lds + c * SC + j2 * kMfmaRow + 16u * ks + 4u * g);
-                        b[ks] ^= (v4i)(int)0x80808080;
+                        b[ks] = *reinterpret_cast<const v4i*>(col + u * kMfmaRow + 16u * ks + 4u * g);
+                        if (stage_a) b[ks] ^= (v4i)(int)0x80808080;
                     }
                     v4i acc[4];
 #pragma unroll
@@ -181,37 +189,25 @@ __device__ __forceinline__ void mfma_tile(const PassJob& J, uint32_t tile_in_job
                     }
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
-                        const uint32_t k1 = 16u * mt + 4u * g + e;
                         const int r = mf_reduce(acc[0][e], acc[1][e], acc[2][e], acc[3][e], qd, qinv);
-                        const uint32_t x = (uint32_t)r + q;  // [0, 2q)
-                        const TwP<uint32_t> t = tw[k1 * 64u + j2];
-                        const uint32_t y = x * t.w - __umulhi(x, t.wp) * q;  // [0, 2q)
-                        ys[k1 * kMfmaRow + j2] = y ^ 0x80808080u;
+                        if (stage_a) {
+                            const uint32_t x = (uint32_t)r + q;  // [0, 2q)
+                            const uint32_t y = x * tv[e].w - __umulhi(x, tv[e].wp) * q;  // [0, 2q)
+                            res[nt][e] = y ^ 0x80808080u;
+                        } else {
+                            res[nt][e] = (uint32_t)(r < 0 ? r + (int)q : r);
+                        }
                     }
                 }
-            } else if (!stage_a && st >= 1) {
-                const uint32_t c = st - 1;
-                const uint32_t* ys = ysb + (c & 1u) * 64u * kMfmaRow;
-                for (uint32_t nt = 0; nt < 4; nt++) {
-                    const uint32_t k1 = 16u * nt + n;
-                    v4i b[4];
+            }
+            __syncthreads();
+            if (active) {
+                uint32_t* col = lds + c * SC;
+                // stage A writes Y[k1][j2]; stage B writes Z[k2][k1]: row 16mt+4g+e, column 16nt+n
 #pragma unroll
-                    for (int ks = 0; ks < 4; ks++) b[ks] = *reinterpret_cast<const v4i*>(ys + k1 * kMfmaRow + 16u * ks + 4u * g);
-                    v4i acc[4];
+                for (uint32_t nt = 0; nt < 4; nt++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        acc[i] = ca[i];
-#pragma unroll
-                        for (int ks = 0; ks < 4; ks++)
-                            acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i][ks], b[ks], acc[i], 0, 0, 0);
-                    }
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        const uint32_t k2 = 16u * mt + 4u * g + e;
-                        const int r = mf_reduce(acc[0][e], acc[1][e], acc[2][e], acc[3][e], qd, qinv);
-                        lds[c * SC + k2 * kMfmaRow + k1] = (uint32_t)(r < 0 ? r + (int)q : r);
-                    }
-                }
+                    for (int e = 0; e < 4; e++) col[(16u * mt + 4u * g + e) * kMfmaRow + 16u * nt + n] = res[nt][e];
             }
             __syncthreads();
         }
@@ -301,8 +297,9 @@ __global__ void __launch_bounds__(512) mfma_pass_kernel(const PassJob* __restric
     const uint32_t lane = threadIdx.x & 63u, mt = (threadIdx.x >> 6) & 3u, g = lane >> 4;
     v4i fa[4][4], cav[4];
     const void* cur = nullptr;
-    const void* tw_cur = nullptr;
-    for (uint32_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    // consecutive tiles (which share cache lines of their column segments) go to
+    // workgroups on the same XCD: workgroup b runs on XCD b % 8
+    for (uint32_t t = xcd_remap(blockIdx.x, gridDim.x); t < total_tiles; t += gridDim.x) {
         uint32_t j = 0;
         if (njobs) {
             uint32_t lo = 0, hi = njobs;
@@ -326,16 +323,6 @@ __global__ void __launch_bounds__(512) mfma_pass_kernel(const PassJob* __restric
 #pragma unroll
                 for (int e = 0; e < 4; e++) cav[i][e] = cr[(stg * 64 + 16 * mt + 4 * g + e) * 4 + i];
             }
-        }
-        if (J.log_r == 12 && J.z != tw_cur) {
-            // stage the 4096-entry twist table once per plan (region after the tile and stage buffers)
-            __syncthreads();
-            tw_cur = J.z;
-            const uint32_t off = ((1u << J.log_c) * mf_col_words(12) + 2u * 64u * kMfmaRow);
-            const uint2* src = reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(J.z) + kMfmaTwistOff);
-            uint2* dst = reinterpret_cast<uint2*>(lds + off);
-            for (uint32_t i = threadIdx.x; i < 4096u; i += blockDim.x) dst[i] = src[i];
-            __syncthreads();
         }
         mfma_tile<Sin, Sout>(J, t - J.tile_begin, lds, fa, cav);
     }

@@ -56,7 +56,7 @@ def test_split_rejects_bad_shapes(nt, ctx):
     with pytest.raises(nt.NttError):
         ctx.transform_split(buf.ptr, BB, 10, 0, [4, 4], 12)      # does not cover n
     with pytest.raises(nt.NttError):
-        ctx.transform_split(buf.ptr, BB, 10, 0, [10], 15)        # tile too large
+        ctx.transform_split(buf.ptr, BB, 10, 0, [10], 16)        # tile too large
 
 
 MFMA_SPLITS = [
