@@ -226,6 +226,36 @@ ntt_status ntt_c_server_replay(ntt_c_server* srv, const ntt_c_desc* polys, const
                                const double* arrival_s, size_t nreq, double* latency_us);
 ntt_status ntt_c_server_destroy(ntt_c_server* srv);
 
+/* ---------------- K-limb big-prime transforms (include/ntt_arbitrary.cuh)
+ * A value is K uint32 limbs of 24 bits, little-endian (DScalar<K>, reference
+ * include/ntt_arbitrary.cuh:51,573-576), 1 <= K <= 12, in Montgomery form
+ * x*R mod p with R = 2^(24K).  `prime_limbs` (K limbs, host memory) is any
+ * odd modulus; `np` = -p^-1 mod 2^24 (mont_np, :456-462).  Arrays are n = 2^log_n
+ * values in natural order, transformed in place; d_tw[k] = w^k * R mod p for a
+ * primitive n-th root w (the table setup_ntt builds, :2698-2793).  Products are
+ * CIOS Montgomery (mont_mul, :1218-1298) with canonical results, so outputs
+ * equal host_ntt_naive (:2651-2676) on the same values, limb for limb. */
+
+/* Forward DIT NTT: run_forward_ct / run_forward_ct_tcu, reference :2547-2646. */
+ntt_status ntt_c_arb_forward(uint32_t* d_data, const uint32_t* d_tw, uint32_t n, uint32_t log_n,
+                             const uint32_t* prime_limbs, uint32_t np, uint32_t K, void* stream);
+/* Inverse NTT (the reference has none): twiddles w^-k = d_tw[(n-k) % n], then a
+ * product by n_inv_mont = n^-1 * R mod p (K limbs, host memory). */
+ntt_status ntt_c_arb_inverse(uint32_t* d_data, const uint32_t* d_tw, uint32_t n, uint32_t log_n,
+                             const uint32_t* prime_limbs, uint32_t np, const uint32_t* n_inv_mont, uint32_t K,
+                             void* stream);
+/* d_a[i] = d_a[i] * d_b[i] * R^-1 mod p, i < n: the pointwise step of a
+ * polynomial product and the psi-twists of the negacyclic transform (no
+ * reference counterpart for K limbs; poly_multiply_ntt, src/ntt_basic.cu:392-409,
+ * is the 31-bit one). */
+ntt_status ntt_c_arb_pointwise_mul(uint32_t* d_a, const uint32_t* d_b, uint32_t n, const uint32_t* prime_limbs,
+                                   uint32_t np, uint32_t K, void* stream);
+/* d_a[i] = d_a[i] * s * R^-1 mod p with s in host memory: into Montgomery form
+ * with s = R^2 mod p, out of it with s = 1 (the conversions of setup_ntt and
+ * run_for_K, reference :2731-2746, 2884-2888). */
+ntt_status ntt_c_arb_scale(uint32_t* d_a, uint32_t n, const uint32_t* scalar_limbs, const uint32_t* prime_limbs,
+                           uint32_t np, uint32_t K, void* stream);
+
 /* ---------------- device-memory and timing helpers for hosts without a HIP
  * binding (tests, benchmarks, FFI callers) ---------------- */
 ntt_status ntt_c_device_count(int* count);

@@ -122,8 +122,18 @@ def lib():
     return L
 
 
-# Later modules of the binding register more prototypes here.
+# Later modules of the binding register more prototypes through register_sigs.
 _EXTRA_SIGS = []
+
+
+def register_sigs(fn):
+    """Adds a function that sets ctypes prototypes on the library.  It runs at
+    once if the library is already loaded, so a module imported after the
+    first lib() call still gets its prototypes (without them ctypes would pass
+    64-bit pointers as C ints)."""
+    _EXTRA_SIGS.append(fn)
+    if _lib is not None:
+        fn(_lib)
 
 
 def check(status):
@@ -376,7 +386,7 @@ def _server_sigs(L):
         f.restype = ctypes.c_int
 
 
-_EXTRA_SIGS.append(_server_sigs)
+register_sigs(_server_sigs)
 
 
 class Server:
@@ -448,7 +458,7 @@ def _tune_sigs(L):
     L.ntt_c_split_for.restype = ctypes.c_int
 
 
-_EXTRA_SIGS.append(_tune_sigs)
+register_sigs(_tune_sigs)
 
 
 def autotune(ctx, q, log_ns, elems_in_flight, reps=5):

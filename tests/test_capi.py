@@ -89,3 +89,21 @@ def test_null_arguments_rejected(nt):
     assert L.ntt_c_bit_reverse(None, 8, None) == 1
     assert L.ntt_c_bit_reverse(ctypes.c_void_p(16), 6, None) == 1
     assert L.ntt_c_rowmaj_to_pretransposed(ctypes.c_void_p(16), ctypes.c_void_p(16), 64, 64, None) == 1
+
+
+def test_every_declared_function_has_a_prototype(nt):
+    """A function without argtypes would get 64-bit pointers as C ints."""
+    import ntt_arb  # noqa: F401  (registers the K-limb prototypes)
+    L = nt.lib()
+    missing = [s for s in nt.header_symbols() if getattr(L, s).argtypes is None]
+    assert not missing, missing
+
+
+def test_prototypes_apply_when_a_module_is_imported_after_load(nt):
+    import importlib
+
+    import ntt_arb
+    L = nt.lib()
+    L.ntt_c_arb_forward.argtypes = None
+    importlib.reload(ntt_arb)
+    assert L.ntt_c_arb_forward.argtypes is not None
