@@ -41,7 +41,12 @@ struct Mod32 {
         return reduce2(x * t.w - hi * q);
     }
     __device__ __forceinline__ V add(V a, V b) const { return reduce2(a + b); }
-    __device__ __forceinline__ V sub(V a, V b) const { return reduce2(a - b + q); }
+    // a - b wraps above 2^32 - q exactly when a < b, and then t + q = a - b + q
+    // is the smaller of the two (2q < 2^32)
+    __device__ __forceinline__ V sub(V a, V b) const {
+        const V t = a - b;
+        return min(t, t + q);
+    }
 };
 
 // q < 2^62: 2q < 2^63, Shoup with a 64-bit quotient.
@@ -67,7 +72,8 @@ enum PostMode : uint32_t { POST_NONE = 0, POST_FOURSTEP = 2, POST_SCALE = 3 };
 // The output form covers the in-place store of the last pass, the transposed
 // store of the first pass and the grouped store of middle passes.
 //   post-multiply exponent  e = (e_a*k*u + e_b*k + e_d*u) mod (e_mask+1),
-//   u = c >> e_shift, factor = lo[e mod 2^lo_bits] * hi[e >> lo_bits].
+//   u = c >> e_shift, factor = lo[e mod 2^lo_bits] * mid[(e >> lo_bits) mod
+//   2^mid_bits] * hi[e >> (lo_bits + mid_bits)] (no mid factor if mid_bits = 0).
 enum JobFlags : uint32_t {
     JF_LOAD_CFAST = 1u,   // consecutive threads walk columns (s_c == 1), else rows (s_r == 1)
     JF_STORE_CFAST = 2u,  // consecutive threads walk columns (o_c == 1), else rows (o_r == 1)
@@ -75,6 +81,8 @@ enum JobFlags : uint32_t {
     JF_Z_LDS = 8u,        // stage the CT table in LDS
     JF_VEC4_ST = 16u,     // 4 consecutive elements per thread on the store side
     JF_MFMA = 32u,        // pass runs on the matrix cores (ntt_mfma.hpp)
+    JF_POST4 = 64u,       // post == POST_FOURSTEP (selects the kernel instantiation)
+    JF_POST_LDS = 128u,   // stage the post-multiply tables in LDS
 };
 
 struct PassJob {
@@ -83,6 +91,7 @@ struct PassJob {
     const void* z;        // TwP<V>[R]: merged CT twiddles, Z[2^l + i]
     const void* tw_lo;    // post-multiply tables (TwP<V>)
     const void* tw_hi;
+    const void* tw_mid;   // middle table of a three-table split (mid_bits > 0)
     uint64_t q;
     uint64_t scale_w, scale_wp;  // POST_SCALE constant (Shoup pair)
     int64_t s_r, s_c;
@@ -92,41 +101,53 @@ struct PassJob {
     uint32_t ncols;              // valid columns per polynomial (last tile may be partial)
     uint32_t tiles_per_poly;
     uint32_t o_lbits, post;
-    uint32_t e_a, e_b, e_d, e_shift, e_mask, lo_bits;
+    uint32_t e_a, e_b, e_d, e_shift, e_mask, lo_bits, mid_bits;
     uint32_t tile_begin;         // first workgroup of this job in the launch
     uint32_t npoly;
     uint32_t flags;
-    uint32_t pad_;
+    // log2 of s_r, s_c, o_r, o_c, o_hs (all powers of two; a zero stride only
+    // ever multiplies a zero term, so its shift is arbitrary)
+    uint8_t sh_sr, sh_sc, sh_or, sh_oc;
+    uint8_t sh_ohs, pad_[3];
 };
 
 // XOR swizzle of the LDS word index: permutes the low 6 bits by higher bits so
 // that both contiguous runs and the bit-reversed read-out hit distinct banks.
 __device__ __forceinline__ uint32_t swz(uint32_t a) { return a ^ (((a >> 6) ^ (a >> 12)) & 63u); }
 
+// Position of entry i of a post-multiply table staged in LDS: XORs bits 5..9
+// into bits 0..4 (a bijection on every power-of-two table size) so that the
+// strided exponents of one wave spread over the banks.
+__device__ __forceinline__ uint32_t tsw(uint32_t i) { return i ^ ((i >> 5) & 31u); }
+
 __device__ __forceinline__ uint32_t bitrev_bits(uint32_t k, uint32_t bits) {
     return bits ? (__brev(k) >> (32u - bits)) : 0u;
 }
 
+// Address arithmetic.  Every map from an element index of a tile to an LDS
+// word (row-major r*C + c, bit-reversed read-out, the XOR swizzle below) is
+// linear over GF(2): map(a ^ b) = map(a) ^ map(b).  A thread's index is
+// lane_bits | uniform_bits (disjoint fields), so an LDS address is
+// map(lane part), computed once per thread, XOR map(uniform part), which the
+// scalar unit computes: one vector XOR per access.  Global offsets are split
+// the same way into a 32-bit per-lane offset and a wave-uniform base pointer.
+
 // m register-resident CT levels [l0, l0+m) for one unit of 2^m elements.
-template <int m, class M, class ZP>
-__device__ __forceinline__ void ct_unit(typename M::V* lds, ZP z, const M& mod, uint32_t u, uint32_t l0,
-                                        uint32_t log_r, uint32_t log_c) {
+// slot0 = LDS byte offset of the unit's first element, vsl[v] = the byte XOR
+// offset of its v-th element; zb[lam] points at the level-(l0+lam) twiddles.
+template <int m, class M>
+__device__ __forceinline__ void ct_unit(unsigned char* lds, const TwP<typename M::V>* const zb[m], const M& mod,
+                                        uint32_t slot0, const uint32_t* vsl) {
     using V = typename M::V;
-    const uint32_t C = 1u << log_c;
-    const uint32_t c = u & (C - 1u);
-    const uint32_t rest = u >> log_c;
-    const uint32_t log_t = log_r - l0 - m;  // log2 of the distance of the last level's pairs
-    const uint32_t s = rest & ((1u << log_t) - 1u);
-    const uint32_t i0 = rest >> log_t;
-    const uint32_t base = (i0 << (log_r - l0)) + s;
+    auto at = [&](uint32_t b) -> V& { return *reinterpret_cast<V*>(lds + b); };
     TwP<V> w[(1 << m) - 1];
 #pragma unroll
     for (int lam = 0; lam < m; lam++)
 #pragma unroll
-        for (int g = 0; g < (1 << lam); g++) w[(1 << lam) - 1 + g] = z[(1u << (l0 + lam)) + (i0 << lam) + g];
+        for (int g = 0; g < (1 << lam); g++) w[(1 << lam) - 1 + g] = zb[lam][g];
     V x[1 << m];
 #pragma unroll
-    for (int v = 0; v < (1 << m); v++) x[v] = lds[swz(((base + ((uint32_t)v << log_t)) << log_c) + c)];
+    for (int v = 0; v < (1 << m); v++) x[v] = at(slot0 ^ vsl[v]);
 #pragma unroll
     for (int lam = 0; lam < m; lam++) {
         const int half = 1 << (m - 1 - lam);
@@ -144,22 +165,46 @@ __device__ __forceinline__ void ct_unit(typename M::V* lds, ZP z, const M& mod, 
         }
     }
 #pragma unroll
-    for (int v = 0; v < (1 << m); v++) lds[swz(((base + ((uint32_t)v << log_t)) << log_c) + c)] = x[v];
+    for (int v = 0; v < (1 << m); v++) at(slot0 ^ vsl[v]) = x[v];
 }
 
+// One round of m levels starting at level l0 over the whole tile.  A unit is
+// one column c and 2^m rows base + v*2^t, t = log_r - l0 - m, base = i0*2^(t+m)
+// + s (s < 2^t); unit index u = (i0 * 2^t + s) * C + c.
+template <int m, class M, class ZP>
+__device__ __forceinline__ void ct_round(typename M::V* lds, ZP z, const M& mod, uint32_t l0, uint32_t log_r,
+                                         uint32_t log_c) {
+    const uint32_t log_t = log_r - l0 - m;
+    const uint32_t units = 1u << (log_r + log_c - m);
+    constexpr uint32_t VB = sizeof(typename M::V);
+    uint32_t vsl[1 << m];
+#pragma unroll
+    for (int v = 0; v < (1 << m); v++) vsl[v] = swz((uint32_t)v << (log_t + log_c)) * VB;
+    const uint32_t lowmask = (1u << (log_t + log_c)) - 1u;
+    for (uint32_t u = threadIdx.x; u < units; u += blockDim.x) {
+        // row-major word of (base, c): u's low log_t + log_c bits stay, i0 moves up by m
+        const uint32_t word = (u & lowmask) | ((u & ~lowmask) << m);
+        const uint32_t i0 = u >> (log_t + log_c);
+        const TwP<typename M::V>* zb[m];
+#pragma unroll
+        for (int lam = 0; lam < m; lam++) zb[lam] = &z[(1u << (l0 + lam)) + (i0 << lam)];
+        ct_unit<m, M>(reinterpret_cast<unsigned char*>(lds), zb, mod, swz(word) * VB, vsl);
+    }
+}
+
+// All log_r levels, in rounds of at most MAXM levels balanced so that no
+// round is much shorter than the others.
 template <int MAXM, class M, class ZP>
 __device__ __forceinline__ void ct_levels(typename M::V* lds, ZP z, const M& mod, uint32_t log_r, uint32_t log_c) {
-    const uint32_t nthreads = blockDim.x;
-    for (uint32_t l0 = 0; l0 < log_r;) {
-        const uint32_t m = min((uint32_t)MAXM, log_r - l0);
-        const uint32_t units = 1u << (log_r + log_c - m);
-        for (uint32_t u = threadIdx.x; u < units; u += nthreads) {
-            switch (m) {
-                case 1: ct_unit<1>(lds, z, mod, u, l0, log_r, log_c); break;
-                case 2: ct_unit<2>(lds, z, mod, u, l0, log_r, log_c); break;
-                case 3: ct_unit<3>(lds, z, mod, u, l0, log_r, log_c); break;
-                default: ct_unit<(MAXM >= 4 ? 4 : 3)>(lds, z, mod, u, l0, log_r, log_c); break;
-            }
+    const uint32_t rounds = (log_r + MAXM - 1) / MAXM;
+    uint32_t l0 = 0;
+    for (uint32_t i = 0; i < rounds; i++) {
+        const uint32_t left = log_r - l0, m = (left + (rounds - i) - 1) / (rounds - i);
+        switch (m) {
+            case 1: ct_round<1>(lds, z, mod, l0, log_r, log_c); break;
+            case 2: ct_round<2>(lds, z, mod, l0, log_r, log_c); break;
+            case 3: ct_round<3>(lds, z, mod, l0, log_r, log_c); break;
+            default: ct_round<(MAXM >= 4 ? 4 : 3)>(lds, z, mod, l0, log_r, log_c); break;
         }
         l0 += m;
         __syncthreads();
@@ -193,7 +238,217 @@ __device__ __forceinline__ void store4<uint64_t>(uint64_t* p, const uint64_t v[4
 // Loads per thread kept in flight before their LDS stores.
 constexpr int kUnroll = 8;
 
-template <class M, class Sin, class Sout, int MAXM>
+// Natural-order store of a tile whose CT output sits at LDS row bitrev(k),
+// with the pass's post-multiply.  Template parameters fix everything that is
+// uniform over a job so that the loop body is straight-line code:
+//   PER  elements per thread and sweep (4: vector store, 1: scalar),
+//   POST 0: none, 1: constant (POST_SCALE), 2: power table (POST_FOURSTEP),
+//   NTAB tables of the power (2 or 3), QM exponent scheme within a quad
+//   (0: per element, 1: one exponent per quad, 2: a fixed step per element),
+//   PRED per-element bounds predicates (partial tiles, tiles smaller than a sweep),
+//   SWT  tables are staged in LDS at tsw(i).
+template <int PER, int POST, int NTAB, int QM, bool PRED, bool SWT, class M, class Sout>
+__device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, unsigned char* lds_raw, uint32_t poly,
+                                           uint32_t col0, const TwP<typename M::V>* tl,
+                                           const TwP<typename M::V>* tm, const TwP<typename M::V>* th) {
+    using V = typename M::V;
+    constexpr uint32_t VB = sizeof(V);
+    const uint32_t log_r = J.log_r, log_c = J.log_c;
+    const uint32_t R = 1u << log_r, C = 1u << log_c;
+    const uint32_t total = R << log_c;
+    const uint32_t ncols = J.ncols;
+    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    auto lds_at = [&](uint32_t byte_off) -> V& { return *reinterpret_cast<V*>(lds_raw + byte_off); };
+    const bool cf = J.flags & JF_STORE_CFAST;
+    const uint32_t sweep = PER * T;
+    const TwP<V> scale{(V)J.scale_w, (V)J.scale_wp};
+    const uint32_t o_lbits = J.o_lbits, omask = (1u << o_lbits) - 1u;
+    const uint32_t sh_or = J.sh_or, sh_oc = J.sh_oc, sh_ohs = J.sh_ohs;
+    const uint32_t e_a = J.e_a, e_b = J.e_b, e_d = J.e_d, e_shift = J.e_shift, e_mask = J.e_mask;
+    const uint32_t b0 = J.lo_bits, b1 = J.mid_bits, b01 = b0 + b1;
+    const uint32_t m0 = (1u << b0) - 1u, m1 = (1u << b1) - 1u;
+    auto expo = [&](uint32_t k, uint32_t cg) -> uint32_t {
+        const uint32_t u = cg >> e_shift;
+        return (e_a * k * u + e_b * k + e_d * u) & e_mask;
+    };
+    auto ti = [&](uint32_t i) -> uint32_t { return SWT ? tsw(i) : i; };
+    auto twmul = [&](V x, uint32_t e) -> V {
+        x = mod.mul(x, tl[ti(e & m0)]);
+        if (NTAB == 3) x = mod.mul(x, tm[ti((e >> b0) & m1)]);
+        return mod.mul(x, th[ti(e >> b01)]);
+    };
+    auto kc = [&](uint32_t i, uint32_t& k, uint32_t& c) {
+        if (cf) { k = i >> log_c; c = i & (C - 1u); }
+        else { c = i >> log_r; k = i & (R - 1u); }
+    };
+    // Output element offset of (k, column c of the tile) from the tile base;
+    // additive over disjoint bit fields of k and c (col0, a multiple of C, is
+    // in the base pointer).
+    auto ooff = [&](uint32_t k, uint32_t c) -> uint32_t {
+        return ((c >> o_lbits) << sh_ohs) + ((c & omask) << sh_oc) + (k << sh_or);
+    };
+    auto slot = [&](uint32_t k, uint32_t c) -> uint32_t { return swz((bitrev_bits(k, log_r) << log_c) | c) * VB; };
+    const uint32_t li = tid * PER;
+    uint32_t lk, lc;
+    kc(li, lk, lc);
+    const bool lane_ok = li < total;
+    const uint32_t l_off = ooff(lk, lc), l_slot = slot(lk, lc);
+    uint32_t e_x[PER];  // byte XOR offsets of quad element e (consecutive c, or consecutive k)
+#pragma unroll
+    for (int e = 0; e < PER; e++) e_x[e] = cf ? (uint32_t)e * VB : swz(bitrev_bits((uint32_t)e, log_r) << log_c) * VB;
+    Sout* __restrict__ out = reinterpret_cast<Sout*>(J.out) + (int64_t)poly * J.out_pstride +
+                             ((int64_t)(col0 >> o_lbits) << sh_ohs) + ((int64_t)(col0 & omask) << sh_oc);
+    for (uint32_t bb = 0; bb < total; bb += kUnroll * sweep) {
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint32_t ui = bb + (uint32_t)u * sweep;
+            if (ui >= total) break;
+            uint32_t uk, uc;
+            kc(ui, uk, uc);
+            const uint32_t c = uc | lc, k = uk | lk, cg = col0 + c;
+            const uint32_t s = l_slot ^ slot(uk, uc);
+            V x[PER];
+#pragma unroll
+            for (int e = 0; e < PER; e++) x[e] = (!PRED || lane_ok) ? lds_at(s ^ e_x[e]) : (V)0;
+            if (POST == 2) {
+                if (QM == 1) {
+                    const uint32_t ex = expo(k, cg);
+                    const TwP<V> f0 = tl[ti(ex & m0)], f2 = th[ti(ex >> b01)];
+                    TwP<V> f1{};
+                    if (NTAB == 3) f1 = tm[ti((ex >> b0) & m1)];
+#pragma unroll
+                    for (int e = 0; e < PER; e++) {
+                        V y = mod.mul(x[e], f0);
+                        if (NTAB == 3) y = mod.mul(y, f1);
+                        x[e] = mod.mul(y, f2);
+                    }
+                } else if (QM == 2) {
+                    uint32_t ex = expo(k, cg);
+                    const uint32_t stp = e_a * (cg >> e_shift) + e_b;
+#pragma unroll
+                    for (int e = 0; e < PER; e++) {
+                        x[e] = twmul(x[e], ex);
+                        ex = (ex + stp) & e_mask;
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < PER; e++)
+                        x[e] = twmul(x[e], expo(cf ? k : k + (uint32_t)e, cf ? cg + (uint32_t)e : cg));
+                }
+            } else if (POST == 1) {
+#pragma unroll
+                for (int e = 0; e < PER; e++) x[e] = mod.mul(x[e], scale);
+            }
+            if (!PRED || (lane_ok && cg < ncols)) {
+                Sout* __restrict__ p = out + (ooff(uk, uc) + l_off);
+                if (PER == 4) {
+                    Sout v[4] = {(Sout)x[0], (Sout)x[PER > 1 ? 1 : 0], (Sout)x[PER > 2 ? 2 : 0], (Sout)x[PER > 3 ? 3 : 0]};
+                    store4<Sout>(p, v);
+                } else {
+                    *p = (Sout)x[0];
+                }
+            }
+        }
+    }
+}
+
+template <int PER, bool P4, bool SWT, class M, class Sout>
+__device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, unsigned char* lds_raw, uint32_t poly,
+                                               uint32_t col0, bool pred, const TwP<typename M::V>* tl,
+                                               const TwP<typename M::V>* tm, const TwP<typename M::V>* th) {
+    const uint32_t post = J.post;
+    if (P4 && post == POST_FOURSTEP) {
+        const bool cf = J.flags & JF_STORE_CFAST;
+        const int qm = PER == 4 ? (cf ? (J.e_shift >= 2 ? 1 : 0) : 2) : 0;
+        const bool three = J.mid_bits != 0;
+#define NTT_STORE4(NT, QM)                                                                          \
+    (pred ? store_loop<PER, 2, NT, QM, true, SWT, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th) \
+          : store_loop<PER, 2, NT, QM, false, SWT, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th))
+        if (three) {
+            if (qm == 1) NTT_STORE4(3, 1);
+            else if (qm == 2) NTT_STORE4(3, 2);
+            else NTT_STORE4(3, 0);
+        } else {
+            if (qm == 1) NTT_STORE4(2, 1);
+            else if (qm == 2) NTT_STORE4(2, 2);
+            else NTT_STORE4(2, 0);
+        }
+#undef NTT_STORE4
+    } else if (post == POST_SCALE) {
+        if (pred) store_loop<PER, 1, 2, 0, true, false, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
+        else store_loop<PER, 1, 2, 0, false, false, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
+    } else {
+        if (pred) store_loop<PER, 0, 2, 0, true, false, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
+        else store_loop<PER, 0, 2, 0, false, false, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
+    }
+}
+
+// Tile load into LDS (row-major, swizzled): PER elements per thread and
+// sweep, PRED per-element bounds predicates.
+template <int PER, bool PRED, class Sin, class V>
+__device__ __forceinline__ void load_loop(const PassJob& J, unsigned char* lds_raw, uint32_t poly, uint32_t col0) {
+    constexpr uint32_t VB = sizeof(V);
+    const uint32_t log_r = J.log_r, log_c = J.log_c;
+    const uint32_t R = 1u << log_r, C = 1u << log_c;
+    const uint32_t total = R << log_c;
+    const uint32_t ncols = J.ncols;
+    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    auto lds_at = [&](uint32_t byte_off) -> V& { return *reinterpret_cast<V*>(lds_raw + byte_off); };
+    const bool cf = J.flags & JF_LOAD_CFAST;
+    const uint32_t sweep = PER * T;
+    const uint32_t sh_sr = J.sh_sr, sh_sc = J.sh_sc;
+    auto rc = [&](uint32_t i, uint32_t& r, uint32_t& c) {
+        if (cf) { r = i >> log_c; c = i & (C - 1u); }
+        else { c = i >> log_r; r = i & (R - 1u); }
+    };
+    // element offset of (r, c) from the tile's first column: < 2^29
+    auto goff = [&](uint32_t r, uint32_t c) -> uint32_t { return (r << sh_sr) + (c << sh_sc); };
+    auto slot = [&](uint32_t r, uint32_t c) -> uint32_t { return swz((r << log_c) | c) * VB; };
+    const uint32_t li = tid * PER;
+    uint32_t lr, lc;
+    rc(li, lr, lc);
+    const bool lane_ok = li < total;
+    const uint32_t l_off = goff(lr, lc), l_slot = slot(lr, lc);
+    uint32_t e_x[PER];  // byte XOR offsets of the elements of a quad
+#pragma unroll
+    for (int e = 0; e < PER; e++) e_x[e] = cf ? (uint32_t)e * VB : swz((uint32_t)e << log_c) * VB;
+    const Sin* __restrict__ in = reinterpret_cast<const Sin*>(J.in) + (int64_t)poly * J.in_pstride +
+                                 ((int64_t)col0 << sh_sc);
+    for (uint32_t bb = 0; bb < total; bb += kUnroll * sweep) {
+        Sin v[kUnroll][PER];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint32_t ui = bb + (uint32_t)u * sweep;
+            if (ui >= total) break;
+            uint32_t ur, uc;
+            rc(ui, ur, uc);
+            const Sin* __restrict__ p = in + (goff(ur, uc) + l_off);
+            const bool ok = !PRED || (lane_ok && col0 + (uc | lc) < ncols);
+            if (PER == 4) {
+                if (ok) load4<Sin>(p, v[u]);
+                else {
+#pragma unroll
+                    for (int e = 0; e < PER; e++) v[u][e] = 0;
+                }
+            } else {
+                v[u][0] = ok ? *p : (Sin)0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint32_t ui = bb + (uint32_t)u * sweep;
+            if (ui >= total) break;
+            if (PRED && !lane_ok) continue;
+            uint32_t ur, uc;
+            rc(ui, ur, uc);
+            const uint32_t s = l_slot ^ slot(ur, uc);
+#pragma unroll
+            for (int e = 0; e < PER; e++) lds_at(s ^ e_x[e]) = (V)v[u][e];
+        }
+    }
+}
+
+template <class M, class Sin, class Sout, int MAXM, bool P4>
 __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job, unsigned char* lds_raw) {
     using V = typename M::V;
     V* lds = reinterpret_cast<V*>(lds_raw);
@@ -204,138 +459,59 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job,
     const uint32_t col0 = tile << log_c;
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
+    const bool full = col0 + C <= ncols;  // every column of the tile exists
     const uint32_t T = blockDim.x, tid = threadIdx.x;
-    const Sin* __restrict__ in = reinterpret_cast<const Sin*>(J.in) + (int64_t)poly * J.in_pstride;
-    Sout* __restrict__ out = reinterpret_cast<Sout*>(J.out) + (int64_t)poly * J.out_pstride;
     const M mod{(decltype(M::q))J.q};
     const uint32_t flags = J.flags;
 
-    // CT table: staged in LDS behind the tile when it fits, else read from global.
-    TwP<V>* zl = reinterpret_cast<TwP<V>*>(lds_raw + (size_t)total * sizeof(V));
+    // Tables staged in LDS behind the tile: the CT table (R entries), then the
+    // post-multiply tables (lo, mid, hi).
+    unsigned char* after = lds_raw + (size_t)total * sizeof(V);
+    TwP<V>* zl = reinterpret_cast<TwP<V>*>(after);
     const TwP<V>* zg = reinterpret_cast<const TwP<V>*>(J.z);
-    if (flags & JF_Z_LDS)
+    if (flags & JF_Z_LDS) {
         for (uint32_t i = tid; i < R; i += T) zl[i] = zg[i];
+        after += (size_t)R * sizeof(TwP<V>);
+    }
+    const uint32_t n_lo = 1u << J.lo_bits, n_mid = J.mid_bits ? 1u << J.mid_bits : 0u;
+    const uint32_t n_hi = (J.e_mask >> (J.lo_bits + J.mid_bits)) + 1u;
+    TwP<V>* pl = reinterpret_cast<TwP<V>*>(after);
+    TwP<V>* pm = pl + n_lo;
+    TwP<V>* ph = pm + n_mid;
+    if (P4 && (flags & JF_POST_LDS)) {
+        const TwP<V>* gl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
+        const TwP<V>* gm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
+        const TwP<V>* gh = reinterpret_cast<const TwP<V>*>(J.tw_hi);
+        for (uint32_t i = tid; i < n_lo; i += T) pl[tsw(i)] = gl[i];
+        for (uint32_t i = tid; i < n_mid; i += T) pm[tsw(i)] = gm[i];
+        for (uint32_t i = tid; i < n_hi; i += T) ph[tsw(i)] = gh[i];
+    }
 
     // ---- load ----
+    // Predicates are needed for partial tiles and for tiles smaller than one sweep.
+    const bool pred_l = !full || total < ((flags & JF_VEC4) ? 4u : 1u) * T;
     if (flags & JF_VEC4) {
-        const uint32_t nq = total >> 2;
-        for (uint32_t q0 = tid; q0 < nq; q0 += kUnroll * T) {
-            Sin v[kUnroll][4];
-            uint32_t rr[kUnroll], cc[kUnroll];
-            bool ok[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                const uint32_t qi = q0 + u * T;
-                const uint32_t idx = qi << 2;
-                if (flags & JF_LOAD_CFAST) { rr[u] = idx >> log_c; cc[u] = idx & (C - 1u); }
-                else { cc[u] = idx >> log_r; rr[u] = idx & (R - 1u); }
-                ok[u] = qi < nq && col0 + cc[u] < ncols;
-                if (ok[u]) load4<Sin>(in + (int64_t)rr[u] * J.s_r + (int64_t)(col0 + cc[u]) * J.s_c, v[u]);
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                if (q0 + u * T >= nq) continue;
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const uint32_t r = rr[u] + ((flags & JF_LOAD_CFAST) ? 0u : (uint32_t)e);
-                    const uint32_t c = cc[u] + ((flags & JF_LOAD_CFAST) ? (uint32_t)e : 0u);
-                    lds[swz((r << log_c) + c)] = ok[u] ? (V)v[u][e] : (V)0;
-                }
-            }
-        }
+        if (pred_l) load_loop<4, true, Sin, V>(J, lds_raw, poly, col0);
+        else load_loop<4, false, Sin, V>(J, lds_raw, poly, col0);
     } else {
-        for (uint32_t i0 = tid; i0 < total; i0 += kUnroll * T) {
-            Sin v[kUnroll];
-            uint32_t li[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                const uint32_t idx = i0 + u * T;
-                uint32_t r, c;
-                if (flags & JF_LOAD_CFAST) { r = idx >> log_c; c = idx & (C - 1u); }
-                else { c = idx >> log_r; r = idx & (R - 1u); }
-                li[u] = (r << log_c) + c;
-                v[u] = 0;
-                if (idx < total && col0 + c < ncols) v[u] = in[(int64_t)r * J.s_r + (int64_t)(col0 + c) * J.s_c];
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++)
-                if (i0 + u * T < total) lds[swz(li[u])] = (V)v[u];
-        }
+        if (pred_l) load_loop<1, true, Sin, V>(J, lds_raw, poly, col0);
+        else load_loop<1, false, Sin, V>(J, lds_raw, poly, col0);
     }
     __syncthreads();
 
     if (flags & JF_Z_LDS) ct_levels<MAXM>(lds, (const TwP<V>*)zl, mod, log_r, log_c);
     else ct_levels<MAXM>(lds, zg, mod, log_r, log_c);
 
-    // ---- store natural order: output row k lives at LDS row bitrev(k) ----
-    const TwP<V>* tlo = reinterpret_cast<const TwP<V>*>(J.tw_lo);
-    const TwP<V>* thi = reinterpret_cast<const TwP<V>*>(J.tw_hi);
-    const uint32_t post_mode = J.post;
-    const TwP<V> scale{(V)J.scale_w, (V)J.scale_wp};
-    const uint32_t lomask = (1u << J.lo_bits) - 1u;
-    const uint32_t omask = (1u << J.o_lbits) - 1u;
-    auto post = [&](V v, uint32_t k, uint32_t cg) -> V {
-        if (post_mode == POST_FOURSTEP) {
-            const uint32_t u = cg >> J.e_shift;
-            const uint32_t e = (J.e_a * k * u + J.e_b * k + J.e_d * u) & J.e_mask;
-            v = mod.mul(v, tlo[e & lomask]);
-            return mod.mul(v, thi[e >> J.lo_bits]);
-        }
-        if (post_mode == POST_SCALE) return mod.mul(v, scale);
-        return v;
-    };
-    auto oaddr = [&](uint32_t k, uint32_t cg) -> int64_t {
-        return (int64_t)(cg >> J.o_lbits) * J.o_hs + (int64_t)(cg & omask) * J.o_c + (int64_t)k * J.o_r;
-    };
-    if (flags & JF_VEC4_ST) {
-        const uint32_t nq = total >> 2;
-        for (uint32_t q0 = tid; q0 < nq; q0 += kUnroll * T) {
-            // Phase 1: positions, LDS reads and post-multiply for kUnroll quads;
-            // phase 2: the stores.  Keeps the table loads of all quads in flight.
-            Sout v[kUnroll][4];
-            int64_t a[kUnroll];
-            bool ok[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                const uint32_t qi = q0 + u * T;
-                const uint32_t idx = qi << 2;
-                uint32_t k, c;
-                if (flags & JF_STORE_CFAST) { k = idx >> log_c; c = idx & (C - 1u); }
-                else { c = idx >> log_r; k = idx & (R - 1u); }
-                ok[u] = qi < nq && col0 + c < ncols;
-                a[u] = oaddr(k, col0 + c);
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const uint32_t kk = k + ((flags & JF_STORE_CFAST) ? 0u : (uint32_t)e);
-                    const uint32_t ce = c + ((flags & JF_STORE_CFAST) ? (uint32_t)e : 0u);
-                    v[u][e] = ok[u] ? (Sout)post(lds[swz((bitrev_bits(kk & (R - 1u), log_r) << log_c) + (ce & (C - 1u)))],
-                                                 kk, col0 + ce)
-                                    : (Sout)0;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++)
-                if (ok[u]) store4<Sout>(out + a[u], v[u]);
-        }
+    const bool pred_s = !full || total < ((flags & JF_VEC4_ST) ? 4u : 1u) * T;
+    const TwP<V>* tl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
+    const TwP<V>* tm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
+    const TwP<V>* th = reinterpret_cast<const TwP<V>*>(J.tw_hi);
+    if (P4 && (flags & JF_POST_LDS)) {
+        if (flags & JF_VEC4_ST) store_dispatch<4, P4, true, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, pl, pm, ph);
+        else store_dispatch<1, P4, true, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, pl, pm, ph);
     } else {
-        for (uint32_t i0 = tid; i0 < total; i0 += kUnroll * T) {
-            V v[kUnroll];
-            int64_t a[kUnroll];
-            bool ok[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                const uint32_t idx = i0 + u * T;
-                uint32_t k, c;
-                if (flags & JF_STORE_CFAST) { k = idx >> log_c; c = idx & (C - 1u); }
-                else { c = idx >> log_r; k = idx & (R - 1u); }
-                ok[u] = idx < total && col0 + c < ncols;
-                a[u] = oaddr(k, col0 + c);
-                v[u] = ok[u] ? post(lds[swz((bitrev_bits(k, log_r) << log_c) + c)], k, col0 + c) : (V)0;
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++)
-                if (ok[u]) out[a[u]] = (Sout)v[u];
-        }
+        if (flags & JF_VEC4_ST) store_dispatch<4, P4, false, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, tl, tm, th);
+        else store_dispatch<1, P4, false, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, tl, tm, th);
     }
 }
 
@@ -350,12 +526,12 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 
 // A launch runs either the single job passed by value (njobs == 0) or one
 // job per workgroup range from a device array sorted by tile_begin.
-template <class M, class Sin, class Sout, int MAXM, int NT>
+template <class M, class Sin, class Sout, int MAXM, int NT, bool P4>
 __global__ void __launch_bounds__(NT) pass_kernel(const PassJob* __restrict__ jobs, uint32_t njobs, PassJob inl) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     if (njobs == 0) {
-        run_pass<M, Sin, Sout, MAXM>(inl, b, lds_raw);
+        run_pass<M, Sin, Sout, MAXM, P4>(inl, b, lds_raw);
         return;
     }
     uint32_t lo = 0, hi = njobs;
@@ -364,7 +540,7 @@ __global__ void __launch_bounds__(NT) pass_kernel(const PassJob* __restrict__ jo
         if (jobs[mid].tile_begin <= b) lo = mid; else hi = mid;
     }
     const PassJob J = jobs[lo];
-    run_pass<M, Sin, Sout, MAXM>(J, b - J.tile_begin, lds_raw);
+    run_pass<M, Sin, Sout, MAXM, P4>(J, b - J.tile_begin, lds_raw);
 }
 
 // ---- small elementwise kernels ----

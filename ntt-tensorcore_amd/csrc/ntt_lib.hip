@@ -173,8 +173,8 @@ struct PassSpec {
     bool mfma = false;                         // z_off then points at an MFMA table block
     int z_off = -1;
     uint32_t post = POST_NONE;
-    int lo_off = -1, hi_off = -1;
-    uint32_t e_a = 0, e_b = 0, e_d = 0, e_shift = 0, e_mask = 0, lo_bits = 0;
+    int lo_off = -1, hi_off = -1, mid_off = -1;
+    uint32_t e_a = 0, e_b = 0, e_d = 0, e_shift = 0, e_mask = 0, lo_bits = 0, mid_bits = 0;
     uint64_t scale_w = 0, scale_wp = 0;
 };
 
@@ -233,12 +233,26 @@ struct TableBuilder {
         return (int)off;
     }
     // Two-level table of base^e, e < 2^mbits: low 2^lob entries and high part.
+    // base^e * mult for e < 2^mbits as a product of table entries: two tables
+    // of <= 2^9 entries up to 18 bits, else three (<= 2^9 each up to 27 bits),
+    // small enough to stage in LDS.
     void pow_tables(PassSpec& s, uint64_t base, uint32_t mbits, uint64_t mult) {
-        const uint32_t lob = (mbits + 1) / 2;
         s.e_mask = (uint32_t)((1ull << mbits) - 1);
-        s.lo_bits = lob;
-        s.lo_off = add(powers(base, 1ull << lob, q));
-        s.hi_off = add(powers(powmod(base, 1ull << lob, q), 1ull << (mbits - lob), q, mult));
+        if (mbits <= 18) {
+            const uint32_t lob = (mbits + 1) / 2;
+            s.lo_bits = lob;
+            s.mid_bits = 0;
+            s.lo_off = add(powers(base, 1ull << lob, q));
+            s.mid_off = -1;
+            s.hi_off = add(powers(powmod(base, 1ull << lob, q), 1ull << (mbits - lob), q, mult));
+            return;
+        }
+        const uint32_t b0 = (mbits + 2) / 3, b1 = (mbits - b0 + 1) / 2, b2 = mbits - b0 - b1;
+        s.lo_bits = b0;
+        s.mid_bits = b1;
+        s.lo_off = add(powers(base, 1ull << b0, q));
+        s.mid_off = add(powers(powmod(base, 1ull << b0, q), 1ull << b1, q));
+        s.hi_off = add(powers(powmod(base, 1ull << (b0 + b1), q), 1ull << b2, q, mult));
     }
 };
 
@@ -409,6 +423,11 @@ static uint64_t plan_scratch_elems(const Plan& P) { return P.npass >= 2 ? (1ull 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 // Load/store order, vector width and LDS staging flags of a job.
+// Entries of the post-multiply tables (lo, mid, hi) of a POST_FOURSTEP job.
+static uint32_t post_table_entries(const PassJob& J) {
+    return (1u << J.lo_bits) + (J.mid_bits ? 1u << J.mid_bits : 0u) + (J.e_mask >> (J.lo_bits + J.mid_bits)) + 1u;
+}
+
 static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma) {
     const uint32_t R = 1u << J.log_r, C = 1u << J.log_c;
     uint32_t f = 0;
@@ -425,6 +444,7 @@ static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma)
         if (scf) vs = vs && C >= 4 && (J.o_r % 4) == 0 && (J.ncols % 4) == 0;
         else vs = vs && J.o_r == 1 && (J.o_c % 4 == 0 || J.o_lbits == 0);
         if (vs) f |= JF_VEC4_ST;
+        if (J.post == POST_FOURSTEP) f |= JF_POST4;
         J.flags = f | JF_MFMA;
         return;
     }
@@ -442,7 +462,15 @@ static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma)
     else vs = vs && J.o_r == 1 && R >= 4 && (J.o_hs % 4) == 0 && (J.o_c % 4 == 0 || J.o_lbits == 0);
     if (vs) f |= JF_VEC4_ST;
     const uint32_t tile = (R << J.log_c) * vbytes(ar);
-    if (tile + R * 2 * vbytes(ar) <= kLdsMax) f |= JF_Z_LDS;
+    uint32_t used = tile;
+    if (tile + R * 2 * vbytes(ar) <= kLdsMax) {
+        f |= JF_Z_LDS;
+        used += R * 2 * vbytes(ar);
+    }
+    if (J.post == POST_FOURSTEP) {
+        f |= JF_POST4;
+        if (used + post_table_entries(J) * 2 * vbytes(ar) <= kLdsMax) f |= JF_POST_LDS;
+    }
     J.flags = f;
 }
 
@@ -454,11 +482,36 @@ static uint32_t job_lds(const PassJob& J, Arith ar) {
     }
     uint32_t b = (R << J.log_c) * vbytes(ar);
     if (J.flags & JF_Z_LDS) b += R * 2 * vbytes(ar);
+    if (J.flags & JF_POST_LDS) b += post_table_entries(J) * 2 * vbytes(ar);
     return b;
 }
 
+// Launch kind of a job: which kernel instantiation runs it.
+static uint32_t job_kind(const PassJob& J) { return J.flags & (JF_MFMA | JF_POST4); }
+
 // Fills the PassJobs of `npoly` polynomials of plan P at data (stride n
 // elements), with workspace at `scratch` (stride n, arithmetic width).
+// log2 of every stride of a job.  make_jobs only makes strides that are powers
+// of two (N >> lr, 1, N, 2^ll, 2^(lr+ll)) or zero where the stride only
+// multiplies a term that is always zero; anything else is a bug in this file.
+static uint8_t stride_shift(int64_t s) {
+    if (s <= 0) return 0;
+    uint8_t k = 0;
+    while (((int64_t)1 << k) < s) k++;
+    if (((int64_t)1 << k) != s) {
+        std::fprintf(stderr, "ntt: internal error: stride %lld is not a power of two\n", (long long)s);
+        std::abort();
+    }
+    return k;
+}
+static void set_shifts(PassJob& J) {
+    J.sh_sr = stride_shift(J.s_r);
+    J.sh_sc = stride_shift(J.s_c);
+    J.sh_or = stride_shift(J.o_r);
+    J.sh_oc = stride_shift(J.o_c);
+    J.sh_ohs = stride_shift(J.o_hs);
+}
+
 static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, PassJob out[4],
                       Store sin_out[4][2]) {
     const uint32_t log_n = P.log_n;
@@ -474,13 +527,14 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         J.z = blk + s.z_off;
         J.tw_lo = s.lo_off >= 0 ? blk + s.lo_off : nullptr;
         J.tw_hi = s.hi_off >= 0 ? blk + s.hi_off : nullptr;
+        J.tw_mid = s.mid_off >= 0 ? blk + s.mid_off : nullptr;
         J.q = P.q;
         J.scale_w = s.scale_w;
         J.scale_wp = s.scale_wp;
         J.log_r = s.log_r;
         J.post = s.post;
         J.e_a = s.e_a; J.e_b = s.e_b; J.e_d = s.e_d;
-        J.e_shift = s.e_shift; J.e_mask = s.e_mask; J.lo_bits = s.lo_bits;
+        J.e_shift = s.e_shift; J.e_mask = s.e_mask; J.lo_bits = s.lo_bits; J.mid_bits = s.mid_bits;
         const bool last = i == P.npass - 1;
         void* dst = last ? data : (src == data ? scratch : data);
         const Store dst_st = last ? st : (src == data ? mid : st);
@@ -509,6 +563,7 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
             J.npoly = (uint32_t)npoly;
             J.in_pstride = J.out_pstride = N;
         }
+        set_shifts(J);
         finish_flags(J, P.arith, src_st, dst_st, s.mfma);
         src = dst;
         src_st = dst_st;
@@ -517,10 +572,10 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
 
 static uint32_t job_tiles(const PassJob& J) { return J.tiles_per_poly * (J.npoly ? J.npoly : 1); }
 
-template <class M, class Sin, class Sout>
+template <class M, class Sin, class Sout, bool P4>
 static ntt_status launch_t(const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t nblocks, uint32_t lds,
                            hipStream_t stream) {
-    auto fn = pass_kernel<M, Sin, Sout, 4, kThreads>;
+    auto fn = pass_kernel<M, Sin, Sout, 4, kThreads, P4>;
     if (lds > 65536) {
         static std::once_flag once;
         std::call_once(once, [&] {
@@ -554,22 +609,34 @@ static ntt_status launch_mfma_t(const PassJob* djobs, uint32_t njobs, const Pass
 }
 
 // One launch of the pass kernel for (arith, input storage, output storage).
-static ntt_status launch_pass(Arith ar, Store sin, Store sout, bool mfma, const PassJob* djobs, uint32_t njobs,
+template <bool P4>
+static ntt_status launch_valu(Arith ar, Store sin, Store sout, const PassJob* djobs, uint32_t njobs,
+                              const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream) {
+    if (ar == Arith::M32) {
+        if (sin == Store::U32 && sout == Store::U32)
+            return launch_t<Mod32, uint32_t, uint32_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+        if (sin == Store::U64 && sout == Store::U32)
+            return launch_t<Mod32, uint64_t, uint32_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+        if (sin == Store::U32 && sout == Store::U64)
+            return launch_t<Mod32, uint32_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+        return launch_t<Mod32, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+    }
+    return launch_t<Mod64, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+}
+
+// One launch of the pass kernel for (arith, input storage, output storage,
+// kind = JF_MFMA / JF_POST4 bits of its jobs).
+static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, const PassJob* djobs, uint32_t njobs,
                               const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream) {
     if (nblocks == 0) return NTT_OK;
-    if (mfma) {
+    if (kind & JF_MFMA) {
         if (sin == Store::U32 && sout == Store::U32) return launch_mfma_t<uint32_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
         if (sin == Store::U64 && sout == Store::U32) return launch_mfma_t<uint64_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
         if (sin == Store::U32 && sout == Store::U64) return launch_mfma_t<uint32_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
         return launch_mfma_t<uint64_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
     }
-    if (ar == Arith::M32) {
-        if (sin == Store::U32 && sout == Store::U32) return launch_t<Mod32, uint32_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
-        if (sin == Store::U64 && sout == Store::U32) return launch_t<Mod32, uint64_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
-        if (sin == Store::U32 && sout == Store::U64) return launch_t<Mod32, uint32_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
-        return launch_t<Mod32, uint64_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
-    }
-    return launch_t<Mod64, uint64_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
+    if (kind & JF_POST4) return launch_valu<true>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream);
+    return launch_valu<false>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream);
 }
 
 // Runs plan P on npoly polynomials at data (contiguous, stride n).
@@ -578,7 +645,7 @@ static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, 
     Store io[4][2];
     make_jobs(P, data, st, npoly, scratch, jobs, io);
     for (int i = 0; i < P.npass; i++) {
-        ntt_status s = launch_pass(P.arith, io[i][0], io[i][1], (jobs[i].flags & JF_MFMA) != 0, nullptr, 0, jobs[i],
+        ntt_status s = launch_pass(P.arith, io[i][0], io[i][1], job_kind(jobs[i]), nullptr, 0, jobs[i],
                                    job_tiles(jobs[i]),
                                    job_lds(jobs[i], P.arith), stream);
         if (s != NTT_OK) return s;
@@ -926,7 +993,7 @@ struct ntt_c_batch {
     struct Group {
         Arith ar;
         Store sin, sout;
-        bool mfma = false;
+        uint32_t kind = 0;  // JF_MFMA / JF_POST4 bits shared by the group's jobs
         uint32_t njobs = 0, nblocks = 0, lds = 0;
         PassJob* djobs = nullptr;
         PassJob inl{};
@@ -988,9 +1055,9 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
     // Group by (phase, arith, storage); within a group order by LDS size so
     // that big tiles start first.
     std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
-        return std::make_tuple(a.phase, (int)a.ar, (int)a.sin, (int)a.sout, (int)(a.job.flags & JF_MFMA),
+        return std::make_tuple(a.phase, (int)a.ar, (int)a.sin, (int)a.sout, job_kind(a.job),
                                -(int)(a.job.log_r + a.job.log_c)) <
-               std::make_tuple(b.phase, (int)b.ar, (int)b.sin, (int)b.sout, (int)(b.job.flags & JF_MFMA),
+               std::make_tuple(b.phase, (int)b.ar, (int)b.sin, (int)b.sout, job_kind(b.job),
                                -(int)(b.job.log_r + b.job.log_c));
     });
     for (size_t i = 0; i < items.size();) {
@@ -1000,12 +1067,11 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
         memset(&g.inl, 0, sizeof(g.inl));
         g.sin = items[i].sin;
         g.sout = items[i].sout;
-        g.mfma = (items[i].job.flags & JF_MFMA) != 0;
+        g.kind = job_kind(items[i].job);
         std::vector<PassJob> jobs;
         uint32_t blocks = 0, lds = 0;
         while (j < items.size() && items[j].phase == items[i].phase && items[j].ar == g.ar && items[j].sin == g.sin &&
-               items[j].sout == g.sout &&
-               (items[j].job.flags & JF_MFMA) == (items[i].job.flags & JF_MFMA)) {
+               items[j].sout == g.sout && job_kind(items[j].job) == g.kind) {
             PassJob J = items[j].job;
             J.tile_begin = blocks;
             blocks += job_tiles(J);
@@ -1031,7 +1097,7 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
 static ntt_status batch_enqueue(ntt_c_batch* B, hipStream_t stream) {
     for (auto& ph : B->phases)
         for (auto& g : ph) {
-            ntt_status st = launch_pass(g.ar, g.sin, g.sout, g.mfma, g.djobs, g.njobs, g.inl, g.nblocks, g.lds, stream);
+            ntt_status st = launch_pass(g.ar, g.sin, g.sout, g.kind, g.djobs, g.njobs, g.inl, g.nblocks, g.lds, stream);
             if (st != NTT_OK) return st;
         }
     return NTT_OK;

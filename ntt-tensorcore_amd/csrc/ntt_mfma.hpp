@@ -217,6 +217,7 @@ __device__ __forceinline__ void mfma_tile(const PassJob& J, uint32_t tile_in_job
     const Mod32 mod{q};
     const TwP<uint32_t>* tlo = reinterpret_cast<const TwP<uint32_t>*>(J.tw_lo);
     const TwP<uint32_t>* thi = reinterpret_cast<const TwP<uint32_t>*>(J.tw_hi);
+    const TwP<uint32_t>* tmid = reinterpret_cast<const TwP<uint32_t>*>(J.tw_mid);
     const uint32_t post_mode = J.post;
     const TwP<uint32_t> scale{(uint32_t)J.scale_w, (uint32_t)J.scale_wp};
     const uint32_t lomask = (1u << J.lo_bits) - 1u;
@@ -225,7 +226,9 @@ __device__ __forceinline__ void mfma_tile(const PassJob& J, uint32_t tile_in_job
         if (post_mode == POST_FOURSTEP) {
             const uint32_t uu = cg >> J.e_shift;
             const uint32_t e = (J.e_a * k * uu + J.e_b * k + J.e_d * uu) & J.e_mask;
-            return mod.mul(mod.mul(x, tlo[e & lomask]), thi[e >> J.lo_bits]);
+            x = mod.mul(x, tlo[e & lomask]);
+            if (J.mid_bits) x = mod.mul(x, tmid[(e >> J.lo_bits) & ((1u << J.mid_bits) - 1u)]);
+            return mod.mul(x, thi[e >> (J.lo_bits + J.mid_bits)]);
         }
         if (post_mode == POST_SCALE) return mod.mul(x, scale);
         return x;

@@ -38,7 +38,9 @@ def test_uneven_batch_matches_oracle_and_graph_replays(nt, orc, u64):
     descs = [(buf.ptr + int(o), j["q"], j["log_n"], f) for j, o, f in zip(jobs, offs, flags)]
     batch = nt.Batch(ctx, descs)
     info = batch.info()
-    assert info["launches"] <= 6  # one per pass and storage kind
+    # one launch per (pass, storage kind, post-multiply kind): at most 2 passes
+    # here (log n <= 16), 3 storage pairs with u64 I/O, 2 post kinds
+    assert info["launches"] <= 12 and info["launches"] < len(jobs)
     batch.launch()
     nt.device_sync()
     got = _download_polys(nt, buf, offs, [p.size for p in polys], u64)
