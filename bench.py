@@ -35,7 +35,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+RAMP_S = 0.3
+
+
 def timed(fn, steps, warmup, stream):
+    # Bring the GPU to steady clocks first (the step for RAMP_S seconds),
+    # then the W counted warmup steps, then exactly K timed steps.
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < RAMP_S:
+        fn()
+        stream.sync()
     for _ in range(warmup):
         fn()
     stream.sync()

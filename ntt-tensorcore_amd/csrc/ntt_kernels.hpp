@@ -246,14 +246,16 @@ constexpr int kUnroll = 8;
 //   NTAB tables of the power (2 or 3), QM exponent scheme within a quad
 //   (0: per element, 1: one exponent per quad, 2: a fixed step per element),
 //   PRED per-element bounds predicates (partial tiles, tiles smaller than a sweep),
-//   SWT  tables are staged in LDS at tsw(i).
-template <int PER, int POST, int NTAB, int QM, bool PRED, bool SWT, class M, class Sout>
+//   SWT  tables are staged in LDS at tsw(i),
+//   LR, LC the tile shape log2 R, log2 C when fixed at compile time (LR = 0:
+//   taken from the job).
+template <int PER, int POST, int NTAB, int QM, bool PRED, bool SWT, int LR, int LC, class M, class Sout>
 __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, unsigned char* lds_raw, uint32_t poly,
                                            uint32_t col0, const TwP<typename M::V>* tl,
                                            const TwP<typename M::V>* tm, const TwP<typename M::V>* th) {
     using V = typename M::V;
     constexpr uint32_t VB = sizeof(V);
-    const uint32_t log_r = J.log_r, log_c = J.log_c;
+    const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
     const uint32_t R = 1u << log_r, C = 1u << log_c;
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
@@ -352,7 +354,7 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, unsig
     }
 }
 
-template <int PER, bool P4, bool SWT, class M, class Sout>
+template <int PER, bool P4, bool SWT, int LR, int LC, class M, class Sout>
 __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, unsigned char* lds_raw, uint32_t poly,
                                                uint32_t col0, bool pred, const TwP<typename M::V>* tl,
                                                const TwP<typename M::V>* tm, const TwP<typename M::V>* th) {
@@ -362,8 +364,8 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
         const int qm = PER == 4 ? (cf ? (J.e_shift >= 2 ? 1 : 0) : 2) : 0;
         const bool three = J.mid_bits != 0;
 #define NTT_STORE4(NT, QM)                                                                          \
-    (pred ? store_loop<PER, 2, NT, QM, true, SWT, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th) \
-          : store_loop<PER, 2, NT, QM, false, SWT, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th))
+    (pred ? store_loop<PER, 2, NT, QM, true, SWT, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th) \
+          : store_loop<PER, 2, NT, QM, false, SWT, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th))
         if (three) {
             if (qm == 1) NTT_STORE4(3, 1);
             else if (qm == 2) NTT_STORE4(3, 2);
@@ -375,20 +377,20 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
         }
 #undef NTT_STORE4
     } else if (post == POST_SCALE) {
-        if (pred) store_loop<PER, 1, 2, 0, true, false, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
-        else store_loop<PER, 1, 2, 0, false, false, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
+        if (pred) store_loop<PER, 1, 2, 0, true, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
+        else store_loop<PER, 1, 2, 0, false, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
     } else {
-        if (pred) store_loop<PER, 0, 2, 0, true, false, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
-        else store_loop<PER, 0, 2, 0, false, false, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
+        if (pred) store_loop<PER, 0, 2, 0, true, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
+        else store_loop<PER, 0, 2, 0, false, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
     }
 }
 
 // Tile load into LDS (row-major, swizzled): PER elements per thread and
-// sweep, PRED per-element bounds predicates.
-template <int PER, bool PRED, class Sin, class V>
+// sweep, PRED per-element bounds predicates, LR/LC as in store_loop.
+template <int PER, bool PRED, int LR, int LC, class Sin, class V>
 __device__ __forceinline__ void load_loop(const PassJob& J, unsigned char* lds_raw, uint32_t poly, uint32_t col0) {
     constexpr uint32_t VB = sizeof(V);
-    const uint32_t log_r = J.log_r, log_c = J.log_c;
+    const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
     const uint32_t R = 1u << log_r, C = 1u << log_c;
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
@@ -448,13 +450,13 @@ __device__ __forceinline__ void load_loop(const PassJob& J, unsigned char* lds_r
     }
 }
 
-template <class M, class Sin, class Sout, int MAXM, bool P4>
+template <class M, class Sin, class Sout, int MAXM, bool P4, int LR, int LC>
 __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job, unsigned char* lds_raw) {
     using V = typename M::V;
     V* lds = reinterpret_cast<V*>(lds_raw);
     const uint32_t poly = tile_in_job / J.tiles_per_poly;
     const uint32_t tile = tile_in_job - poly * J.tiles_per_poly;
-    const uint32_t log_r = J.log_r, log_c = J.log_c;
+    const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
     const uint32_t R = 1u << log_r, C = 1u << log_c;
     const uint32_t col0 = tile << log_c;
     const uint32_t total = R << log_c;
@@ -490,12 +492,14 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job,
     // ---- load ----
     // Predicates are needed for partial tiles and for tiles smaller than one sweep.
     const bool pred_l = !full || total < ((flags & JF_VEC4) ? 4u : 1u) * T;
-    if (flags & JF_VEC4) {
-        if (pred_l) load_loop<4, true, Sin, V>(J, lds_raw, poly, col0);
-        else load_loop<4, false, Sin, V>(J, lds_raw, poly, col0);
+    if (LR) {  // shape-specialised launches are full tiles with vector loads
+        load_loop<4, false, LR, LC, Sin, V>(J, lds_raw, poly, col0);
+    } else if (flags & JF_VEC4) {
+        if (pred_l) load_loop<4, true, LR, LC, Sin, V>(J, lds_raw, poly, col0);
+        else load_loop<4, false, LR, LC, Sin, V>(J, lds_raw, poly, col0);
     } else {
-        if (pred_l) load_loop<1, true, Sin, V>(J, lds_raw, poly, col0);
-        else load_loop<1, false, Sin, V>(J, lds_raw, poly, col0);
+        if (pred_l) load_loop<1, true, LR, LC, Sin, V>(J, lds_raw, poly, col0);
+        else load_loop<1, false, LR, LC, Sin, V>(J, lds_raw, poly, col0);
     }
     __syncthreads();
 
@@ -506,12 +510,14 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job,
     const TwP<V>* tl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
     const TwP<V>* tm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
     const TwP<V>* th = reinterpret_cast<const TwP<V>*>(J.tw_hi);
-    if (P4 && (flags & JF_POST_LDS)) {
-        if (flags & JF_VEC4_ST) store_dispatch<4, P4, true, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, pl, pm, ph);
-        else store_dispatch<1, P4, true, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, pl, pm, ph);
+    if (LR) {  // shape-specialised: vector stores, full tiles, LDS post tables
+        store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, false, pl, pm, ph);
+    } else if (P4 && (flags & JF_POST_LDS)) {
+        if (flags & JF_VEC4_ST) store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, pl, pm, ph);
+        else store_dispatch<1, P4, true, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, pl, pm, ph);
     } else {
-        if (flags & JF_VEC4_ST) store_dispatch<4, P4, false, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, tl, tm, th);
-        else store_dispatch<1, P4, false, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, tl, tm, th);
+        if (flags & JF_VEC4_ST) store_dispatch<4, P4, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, tl, tm, th);
+        else store_dispatch<1, P4, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, tl, tm, th);
     }
 }
 
@@ -526,21 +532,30 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 
 // A launch runs either the single job passed by value (njobs == 0) or one
 // job per workgroup range from a device array sorted by tile_begin.
-template <class M, class Sin, class Sout, int MAXM, int NT, bool P4>
-__global__ void __launch_bounds__(NT) pass_kernel(const PassJob* __restrict__ jobs, uint32_t njobs, PassJob inl) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-    if (njobs == 0) {
-        run_pass<M, Sin, Sout, MAXM, P4>(inl, b, lds_raw);
-        return;
-    }
+__device__ __forceinline__ uint32_t find_job(const PassJob* __restrict__ jobs, uint32_t njobs, uint32_t tile) {
     uint32_t lo = 0, hi = njobs;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].tile_begin <= b) lo = mid; else hi = mid;
+        if (jobs[mid].tile_begin <= tile) lo = mid; else hi = mid;
     }
-    const PassJob J = jobs[lo];
-    run_pass<M, Sin, Sout, MAXM, P4>(J, b - J.tile_begin, lds_raw);
+    return lo;
+}
+
+// A launch runs either the single job passed by value (njobs == 0) or one
+// job per tile range from a device array sorted by tile_begin; one tile per
+// workgroup.
+// LR > 0: every job of the launch has tiles of 2^LR x 2^LC, vector loads and
+// stores, full tiles and (with P4) LDS post tables; see pass_shape_ok().
+template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC>
+__global__ void __launch_bounds__(NT) pass_kernel(const PassJob* __restrict__ jobs, uint32_t njobs, PassJob inl) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (njobs == 0) {
+        run_pass<M, Sin, Sout, MAXM, P4, LR, LC>(inl, tile, lds_raw);
+        return;
+    }
+    const PassJob J = jobs[find_job(jobs, njobs, tile)];
+    run_pass<M, Sin, Sout, MAXM, P4, LR, LC>(J, tile - J.tile_begin, lds_raw);
 }
 
 // ---- small elementwise kernels ----

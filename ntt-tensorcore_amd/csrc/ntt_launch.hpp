@@ -1,0 +1,63 @@
+// ntt_launch.hpp — host launchers of the pass kernel.  The explicit
+// instantiations live in pass_*.hip (one translation unit per storage type,
+// compiled in parallel); ntt_lib.hip only declares them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "ntt_kernels.hpp"
+
+namespace nttk {
+
+constexpr int kPassThreads = 512;
+constexpr uint32_t kPassLdsMax = 81920;  // LDS bytes a pass workgroup may use
+
+// Launches pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC> with one
+// workgroup per tile (LR = 0: the generic kernel).
+template <class M, class Sin, class Sout, bool P4, int LR = 0, int LC = 0>
+hipError_t launch_pass_kernel(const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t ntiles, uint32_t lds,
+                              hipStream_t stream) {
+    auto fn = pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC>;
+    static std::once_flag once;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPassLdsMax);
+    });
+    hipLaunchKernelGGL(fn, dim3(ntiles), dim3(kPassThreads), lds, stream, djobs, njobs, inl);
+    return hipGetLastError();
+}
+
+#define NTT_PASS_INSTANCES(EXT, M, SIN, SOUT)                                                                  \
+    EXT template hipError_t launch_pass_kernel<M, SIN, SOUT, false>(const PassJob*, uint32_t, const PassJob&, \
+                                                                     uint32_t, uint32_t, hipStream_t);        \
+    EXT template hipError_t launch_pass_kernel<M, SIN, SOUT, true>(const PassJob*, uint32_t, const PassJob&,  \
+                                                                    uint32_t, uint32_t, hipStream_t);
+
+// Tile shapes (log2 R, log2 C) with a specialised kernel for 32-bit
+// coefficients: every radix 2^6..2^12 at tiles of 2^13 and 2^14 elements.
+#define NTT_SHAPES_A(X) X(6, 7) X(7, 6) X(8, 5)
+#define NTT_SHAPES_B(X) X(9, 4) X(10, 3) X(11, 2)
+#define NTT_SHAPES_C(X) X(8, 6) X(9, 5) X(10, 4)
+#define NTT_SHAPES_D(X) X(11, 3) X(12, 2)
+#define NTT_SHAPES(X) NTT_SHAPES_A(X) NTT_SHAPES_B(X) NTT_SHAPES_C(X) NTT_SHAPES_D(X)
+
+#define NTT_SHAPE_INSTANCE_EXTERN(LR, LC)                                                                       \
+    extern template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, false, LR, LC>(                    \
+        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);                             \
+    extern template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, true, LR, LC>(                     \
+        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);
+#define NTT_SHAPE_INSTANCE(LR, LC)                                                                              \
+    template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, false, LR, LC>(                           \
+        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);                             \
+    template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, true, LR, LC>(                            \
+        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);
+
+// Instantiated in pass_*.hip.
+NTT_PASS_INSTANCES(extern, Mod32, uint32_t, uint32_t)
+NTT_PASS_INSTANCES(extern, Mod32, uint32_t, uint64_t)
+NTT_PASS_INSTANCES(extern, Mod32, uint64_t, uint32_t)
+NTT_PASS_INSTANCES(extern, Mod32, uint64_t, uint64_t)
+NTT_PASS_INSTANCES(extern, Mod64, uint64_t, uint64_t)
+NTT_SHAPES(NTT_SHAPE_INSTANCE_EXTERN)
+
+}  // namespace nttk

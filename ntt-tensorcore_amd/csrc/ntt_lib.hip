@@ -21,6 +21,7 @@
 #include "../../include/ntt_c.h"
 #include "field_host.hpp"
 #include "ntt_kernels.hpp"
+#include "ntt_launch.hpp"
 #include "ntt_mfma.hpp"
 
 using namespace nttk;
@@ -102,8 +103,8 @@ __global__ void powers_kernel(uint64_t* out, uint64_t n, const TwP<typename M::V
 
 // ------------------------------------------------------------------ device limits
 // LDS budget of one workgroup (tile plus, when it fits, the CT table).
-static const uint32_t kLdsMax = 81920;
-static const int kThreads = 512;
+static const uint32_t kLdsMax = kPassLdsMax;
+static const int kThreads = kPassThreads;
 
 // ------------------------------------------------------------------ plans
 enum class Arith { M32, M64 };
@@ -461,16 +462,19 @@ static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma)
     if (scf) vs = vs && C >= 4 && (J.o_r % 4) == 0 && (J.o_hs % 4) == 0 && (J.ncols % 4) == 0;
     else vs = vs && J.o_r == 1 && R >= 4 && (J.o_hs % 4) == 0 && (J.o_c % 4 == 0 || J.o_lbits == 0);
     if (vs) f |= JF_VEC4_ST;
+    // LDS after the tile: the post tables first (their reads are scattered),
+    // then the CT table (its reads are mostly broadcast) if it still fits.
     const uint32_t tile = (R << J.log_c) * vbytes(ar);
     uint32_t used = tile;
-    if (tile + R * 2 * vbytes(ar) <= kLdsMax) {
-        f |= JF_Z_LDS;
-        used += R * 2 * vbytes(ar);
-    }
     if (J.post == POST_FOURSTEP) {
         f |= JF_POST4;
-        if (used + post_table_entries(J) * 2 * vbytes(ar) <= kLdsMax) f |= JF_POST_LDS;
+        const uint32_t pb = post_table_entries(J) * 2 * vbytes(ar);
+        if (used + pb <= kLdsMax) {
+            f |= JF_POST_LDS;
+            used += pb;
+        }
     }
+    if (used + R * 2 * vbytes(ar) <= kLdsMax) f |= JF_Z_LDS;
     J.flags = f;
 }
 
@@ -572,21 +576,6 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
 
 static uint32_t job_tiles(const PassJob& J) { return J.tiles_per_poly * (J.npoly ? J.npoly : 1); }
 
-template <class M, class Sin, class Sout, bool P4>
-static ntt_status launch_t(const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t nblocks, uint32_t lds,
-                           hipStream_t stream) {
-    auto fn = pass_kernel<M, Sin, Sout, 4, kThreads, P4>;
-    if (lds > 65536) {
-        static std::once_flag once;
-        std::call_once(once, [&] {
-            (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
-        });
-    }
-    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(kThreads), lds, stream, djobs, njobs, inl);
-    HIP_TRY(hipGetLastError());
-    return NTT_OK;
-}
-
 template <class Sin, class Sout>
 static ntt_status launch_mfma_t(const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t ntiles, uint32_t lds,
                                 hipStream_t stream) {
@@ -612,22 +601,25 @@ static ntt_status launch_mfma_t(const PassJob* djobs, uint32_t njobs, const Pass
 template <bool P4>
 static ntt_status launch_valu(Arith ar, Store sin, Store sout, const PassJob* djobs, uint32_t njobs,
                               const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream) {
-    if (ar == Arith::M32) {
-        if (sin == Store::U32 && sout == Store::U32)
-            return launch_t<Mod32, uint32_t, uint32_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
-        if (sin == Store::U64 && sout == Store::U32)
-            return launch_t<Mod32, uint64_t, uint32_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
-        if (sin == Store::U32 && sout == Store::U64)
-            return launch_t<Mod32, uint32_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
-        return launch_t<Mod32, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+    hipError_t e;
+    if (ar == Arith::M32 && sin == Store::U32) {
+        e = sout == Store::U32 ? launch_pass_kernel<Mod32, uint32_t, uint32_t, P4>(djobs, njobs, inl, nblocks, lds, stream)
+                               : launch_pass_kernel<Mod32, uint32_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+    } else if (ar == Arith::M32) {
+        e = sout == Store::U32 ? launch_pass_kernel<Mod32, uint64_t, uint32_t, P4>(djobs, njobs, inl, nblocks, lds, stream)
+                               : launch_pass_kernel<Mod32, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+    } else {
+        e = launch_pass_kernel<Mod64, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
     }
-    return launch_t<Mod64, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+    HIP_TRY(e);
+    return NTT_OK;
 }
 
 // One launch of the pass kernel for (arith, input storage, output storage,
 // kind = JF_MFMA / JF_POST4 bits of its jobs).
 static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, const PassJob* djobs, uint32_t njobs,
-                              const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream) {
+                              const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream,
+                              uint32_t shape = 0) {
     if (nblocks == 0) return NTT_OK;
     if (kind & JF_MFMA) {
         if (sin == Store::U32 && sout == Store::U32) return launch_mfma_t<uint32_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
@@ -635,19 +627,61 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, co
         if (sin == Store::U32 && sout == Store::U64) return launch_mfma_t<uint32_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
         return launch_mfma_t<uint64_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
     }
+    if (shape) {
+        const bool p4 = kind & JF_POST4;
+        hipError_t e = hipErrorInvalidValue;
+        switch (shape) {
+#define NTT_SHAPE_CASE(LR, LC)                                                                                 \
+    case (LR << 8) | LC:                                                                                        \
+        e = p4 ? launch_pass_kernel<Mod32, uint32_t, uint32_t, true, LR, LC>(djobs, njobs, inl, nblocks, lds, stream)  \
+               : launch_pass_kernel<Mod32, uint32_t, uint32_t, false, LR, LC>(djobs, njobs, inl, nblocks, lds, stream); \
+        break;
+            NTT_SHAPES(NTT_SHAPE_CASE)
+#undef NTT_SHAPE_CASE
+            default: return fail(NTT_ERR_DEVICE, "internal: no kernel for this tile shape");
+        }
+        HIP_TRY(e);
+        return NTT_OK;
+    }
     if (kind & JF_POST4) return launch_valu<true>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream);
     return launch_valu<false>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream);
 }
 
+// Shape-specialised kernel for job J, as (log_r << 8) | log_c, or 0: 32-bit
+// coefficients in and out, vector loads and stores, full tiles, post tables
+// in LDS, and a shape that has an instance.
+static uint32_t pass_shape_of(const PassJob& J, Arith ar, Store sin, Store sout) {
+    if (ar != Arith::M32 || sin != Store::U32 || sout != Store::U32) return 0;
+    if ((J.flags & JF_MFMA) || !(J.flags & JF_VEC4) || !(J.flags & JF_VEC4_ST)) return 0;
+    if (J.post == POST_FOURSTEP && !(J.flags & JF_POST_LDS)) return 0;
+    if (J.ncols % (1u << J.log_c) != 0) return 0;
+    const uint32_t key = (J.log_r << 8) | J.log_c;
+    switch (key) {
+#define NTT_SHAPE_KEY(LR, LC) case (LR << 8) | LC:
+        NTT_SHAPES(NTT_SHAPE_KEY)
+#undef NTT_SHAPE_KEY
+        return key;
+        default: return 0;
+    }
+}
+
 // Runs plan P on npoly polynomials at data (contiguous, stride n).
+// The shape-specialised kernel every job of a launch can use, or 0.
+static uint32_t group_shape(const PassJob* jobs, size_t n, Arith ar, Store sin, Store sout) {
+    const uint32_t sh = n ? pass_shape_of(jobs[0], ar, sin, sout) : 0;
+    for (size_t i = 1; i < n && sh; i++)
+        if (pass_shape_of(jobs[i], ar, sin, sout) != sh) return 0;
+    return sh;
+}
+
 static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, hipStream_t stream) {
     PassJob jobs[4];
     Store io[4][2];
     make_jobs(P, data, st, npoly, scratch, jobs, io);
     for (int i = 0; i < P.npass; i++) {
         ntt_status s = launch_pass(P.arith, io[i][0], io[i][1], job_kind(jobs[i]), nullptr, 0, jobs[i],
-                                   job_tiles(jobs[i]),
-                                   job_lds(jobs[i], P.arith), stream);
+                                   job_tiles(jobs[i]), job_lds(jobs[i], P.arith), stream,
+                                   pass_shape_of(jobs[i], P.arith, io[i][0], io[i][1]));
         if (s != NTT_OK) return s;
     }
     return NTT_OK;
@@ -993,7 +1027,8 @@ struct ntt_c_batch {
     struct Group {
         Arith ar;
         Store sin, sout;
-        uint32_t kind = 0;  // JF_MFMA / JF_POST4 bits shared by the group's jobs
+        uint32_t kind = 0;   // JF_MFMA / JF_POST4 bits shared by the group's jobs
+        uint32_t shape = 0;  // shape-specialised kernel shared by all of them (0: generic)
         uint32_t njobs = 0, nblocks = 0, lds = 0;
         PassJob* djobs = nullptr;
         PassJob inl{};
@@ -1080,6 +1115,7 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
             j++;
         }
         g.njobs = (uint32_t)jobs.size();
+        g.shape = group_shape(jobs.data(), jobs.size(), g.ar, g.sin, g.sout);
         g.nblocks = blocks;
         g.lds = lds;
         hipError_t e = hipMalloc(&g.djobs, jobs.size() * sizeof(PassJob));
@@ -1097,7 +1133,8 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
 static ntt_status batch_enqueue(ntt_c_batch* B, hipStream_t stream) {
     for (auto& ph : B->phases)
         for (auto& g : ph) {
-            ntt_status st = launch_pass(g.ar, g.sin, g.sout, g.kind, g.djobs, g.njobs, g.inl, g.nblocks, g.lds, stream);
+            ntt_status st =
+                launch_pass(g.ar, g.sin, g.sout, g.kind, g.djobs, g.njobs, g.inl, g.nblocks, g.lds, stream, g.shape);
             if (st != NTT_OK) return st;
         }
     return NTT_OK;

@@ -1,0 +1,6 @@
+// Pass-kernel instances: 32-bit arithmetic, 32-bit input and 64-bit output.
+#include "ntt_launch.hpp"
+namespace nttk {
+NTT_PASS_INSTANCES(, Mod32, uint32_t, uint64_t)
+NTT_PASS_INSTANCES(, Mod32, uint64_t, uint32_t)
+}
