@@ -223,6 +223,30 @@ def extra_u64_abi(args, stream):
             "frac": 2 * n * 8 / t / 1e9 / HBM_PEAK_GBS}
 
 
+def extra_klimb(stream):
+    """K-limb big-prime path (include/ntt_arbitrary.cuh): forward + inverse over
+    Montgomery-form limbs, n = 2^20 for the 48-bit (K=2) and Goldilocks (K=3)
+    fields, n = 2^16 for BN254 (K=11).  Algorithmic bytes: 2 n 4K per transform."""
+    import ntt_arb as A
+    out = {}
+    for K, log_n in ((2, 20), (3, 20), (11, 16)):
+        n = 1 << log_n
+        f = A.Field(A.FIELDS[K][0], K)
+        plan = A.ArbNtt(f, log_n)
+        rng = np.random.default_rng(K)
+        x = rng.integers(0, 1 << 24, size=(n, K), dtype=np.uint32)
+        x[:, -1] = rng.integers(0, int(f.p_limbs[-1]), size=n, dtype=np.uint32)
+        buf = nt.DeviceBuffer.from_array(x)
+        ms = timed(lambda: (plan.forward(buf, stream), plan.inverse(buf, stream)), 10, 2, stream)
+        stream.sync()
+        ok = np.array_equal(buf.download(np.uint32, n * K).reshape(n, K), x)
+        buf.free()
+        t = ms / 10 / 2 / 1e3
+        out[f"K{K}_2^{log_n}"] = {"us": t * 1e6, "Gbfly_s": W.butterflies(log_n) / t / 1e9,
+                                  "GB_s": 2 * n * 4 * K / t / 1e9, "roundtrip_ok": bool(ok)}
+    return out
+
+
 def extra_serving(ctx, qps_list=(100, 500, 1000, 2000, 4000, 8000), count=120):
     """cfg5: Poisson arrivals, 1..32 polys per request, log2 n in [12, 22],
     BabyBear (uint32) or a 62-bit prime (uint64); device-resident operands."""
@@ -319,6 +343,7 @@ def main():
                          ("rns_24x2_16", lambda: extra_rns(ctx, stream)),
                          ("uneven_512", lambda: extra_uneven(ctx, stream)),
                          ("u64_abi", lambda: extra_u64_abi(args, stream)),
+                         ("klimb", lambda: extra_klimb(stream)),
                          ("serving", lambda: extra_serving(ctx))]:
             log(f"[bench] extra {name}")
             extras[name] = fn()
