@@ -111,9 +111,24 @@ struct PassJob {
     uint8_t sh_ohs, pad_[3];
 };
 
+// Threads of every pass-kernel workgroup (a compile-time constant so that the
+// uniform index arithmetic folds).
+constexpr uint32_t kNT = 512;
+
 // XOR swizzle of the LDS word index: permutes the low 6 bits by higher bits so
 // that both contiguous runs and the bit-reversed read-out hit distinct banks.
 __device__ __forceinline__ uint32_t swz(uint32_t a) { return a ^ (((a >> 6) ^ (a >> 12)) & 63u); }
+
+// Workgroup barrier that orders LDS accesses only: waits for this wave's LDS
+// operations (lgkmcnt 0, vmcnt and expcnt left at their maxima), then
+// s_barrier.  __syncthreads() would also wait for every outstanding global
+// load and store (vmcnt 0), which serialises tile loads in flight with the
+// CT rounds.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
 
 // Position of entry i of a post-multiply table staged in LDS: XORs bits 5..9
 // into bits 0..4 (a bijection on every power-of-two table size) so that the
@@ -181,7 +196,7 @@ __device__ __forceinline__ void ct_round(typename M::V* lds, ZP z, const M& mod,
 #pragma unroll
     for (int v = 0; v < (1 << m); v++) vsl[v] = swz((uint32_t)v << (log_t + log_c)) * VB;
     const uint32_t lowmask = (1u << (log_t + log_c)) - 1u;
-    for (uint32_t u = threadIdx.x; u < units; u += blockDim.x) {
+    for (uint32_t u = threadIdx.x; u < units; u += kNT) {
         // row-major word of (base, c): u's low log_t + log_c bits stay, i0 moves up by m
         const uint32_t word = (u & lowmask) | ((u & ~lowmask) << m);
         const uint32_t i0 = u >> (log_t + log_c);
@@ -207,7 +222,20 @@ __device__ __forceinline__ void ct_levels(typename M::V* lds, ZP z, const M& mod
             default: ct_round<(MAXM >= 4 ? 4 : 3)>(lds, z, mod, l0, log_r, log_c); break;
         }
         l0 += m;
-        __syncthreads();
+        lds_barrier();
+    }
+}
+
+// ct_levels for a tile shape fixed at compile time: the rounds unrolled with
+// constant first levels, so every uniform offset is an immediate.
+template <int LR, int LC, int MAXM, int L0, class M, class ZP>
+__device__ __forceinline__ void ct_levels_c(typename M::V* lds, ZP z, const M& mod) {
+    if constexpr (L0 < LR) {
+        constexpr int rounds = (LR - L0 + MAXM - 1) / MAXM;
+        constexpr int m = (LR - L0 + rounds - 1) / rounds;
+        ct_round<m>(lds, z, mod, (uint32_t)L0, (uint32_t)LR, (uint32_t)LC);
+        lds_barrier();
+        ct_levels_c<LR, LC, MAXM, L0 + m, M, ZP>(lds, z, mod);
     }
 }
 
@@ -259,7 +287,7 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, unsig
     const uint32_t R = 1u << log_r, C = 1u << log_c;
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
-    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    const uint32_t T = kNT, tid = threadIdx.x;
     auto lds_at = [&](uint32_t byte_off) -> V& { return *reinterpret_cast<V*>(lds_raw + byte_off); };
     const bool cf = J.flags & JF_STORE_CFAST;
     const uint32_t sweep = PER * T;
@@ -297,7 +325,8 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, unsig
     const uint32_t l_off = ooff(lk, lc), l_slot = slot(lk, lc);
     uint32_t e_x[PER];  // byte XOR offsets of quad element e (consecutive c, or consecutive k)
 #pragma unroll
-    for (int e = 0; e < PER; e++) e_x[e] = cf ? (uint32_t)e * VB : swz(bitrev_bits((uint32_t)e, log_r) << log_c) * VB;
+    for (int e = 0; e < PER; e++)
+        e_x[e] = cf ? (uint32_t)e * VB : swz(bitrev_bits((uint32_t)e, log_r) << log_c) * VB;
     Sout* __restrict__ out = reinterpret_cast<Sout*>(J.out) + (int64_t)poly * J.out_pstride +
                              ((int64_t)(col0 >> o_lbits) << sh_ohs) + ((int64_t)(col0 & omask) << sh_oc);
     for (uint32_t bb = 0; bb < total; bb += kUnroll * sweep) {
@@ -394,7 +423,7 @@ __device__ __forceinline__ void load_loop(const PassJob& J, unsigned char* lds_r
     const uint32_t R = 1u << log_r, C = 1u << log_c;
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
-    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    const uint32_t T = kNT, tid = threadIdx.x;
     auto lds_at = [&](uint32_t byte_off) -> V& { return *reinterpret_cast<V*>(lds_raw + byte_off); };
     const bool cf = J.flags & JF_LOAD_CFAST;
     const uint32_t sweep = PER * T;
@@ -462,7 +491,7 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job,
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
     const bool full = col0 + C <= ncols;  // every column of the tile exists
-    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    const uint32_t T = kNT, tid = threadIdx.x;
     const M mod{(decltype(M::q))J.q};
     const uint32_t flags = J.flags;
 
@@ -501,10 +530,15 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job,
         if (pred_l) load_loop<1, true, LR, LC, Sin, V>(J, lds_raw, poly, col0);
         else load_loop<1, false, LR, LC, Sin, V>(J, lds_raw, poly, col0);
     }
-    __syncthreads();
+    lds_barrier();
 
-    if (flags & JF_Z_LDS) ct_levels<MAXM>(lds, (const TwP<V>*)zl, mod, log_r, log_c);
-    else ct_levels<MAXM>(lds, zg, mod, log_r, log_c);
+    if (LR) {
+        if (flags & JF_Z_LDS) ct_levels_c<LR, LC, MAXM, 0, M, const TwP<V>*>(lds, (const TwP<V>*)zl, mod);
+        else ct_levels_c<LR, LC, MAXM, 0, M, const TwP<V>*>(lds, zg, mod);
+    } else {
+        if (flags & JF_Z_LDS) ct_levels<MAXM>(lds, (const TwP<V>*)zl, mod, log_r, log_c);
+        else ct_levels<MAXM>(lds, zg, mod, log_r, log_c);
+    }
 
     const bool pred_s = !full || total < ((flags & JF_VEC4_ST) ? 4u : 1u) * T;
     const TwP<V>* tl = reinterpret_cast<const TwP<V>*>(J.tw_lo);

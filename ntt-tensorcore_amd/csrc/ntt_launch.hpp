@@ -10,7 +10,7 @@
 
 namespace nttk {
 
-constexpr int kPassThreads = 512;
+constexpr int kPassThreads = (int)kNT;
 constexpr uint32_t kPassLdsMax = 81920;  // LDS bytes a pass workgroup may use
 
 // Launches pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC> with one
@@ -34,12 +34,16 @@ hipError_t launch_pass_kernel(const PassJob* djobs, uint32_t njobs, const PassJo
                                                                     uint32_t, uint32_t, hipStream_t);
 
 // Tile shapes (log2 R, log2 C) with a specialised kernel for 32-bit
-// coefficients: every radix 2^6..2^12 at tiles of 2^13 and 2^14 elements.
+// coefficients: radix 2^6..2^12 at tiles of 2^12, 2^13 and 2^14 elements
+// (at least 4 columns, for vector loads).
 #define NTT_SHAPES_A(X) X(6, 7) X(7, 6) X(8, 5)
 #define NTT_SHAPES_B(X) X(9, 4) X(10, 3) X(11, 2)
 #define NTT_SHAPES_C(X) X(8, 6) X(9, 5) X(10, 4)
 #define NTT_SHAPES_D(X) X(11, 3) X(12, 2)
-#define NTT_SHAPES(X) NTT_SHAPES_A(X) NTT_SHAPES_B(X) NTT_SHAPES_C(X) NTT_SHAPES_D(X)
+#define NTT_SHAPES_E(X) X(6, 6) X(7, 5) X(8, 4)
+#define NTT_SHAPES_F(X) X(9, 3) X(10, 2)
+#define NTT_SHAPES(X) \
+    NTT_SHAPES_A(X) NTT_SHAPES_B(X) NTT_SHAPES_C(X) NTT_SHAPES_D(X) NTT_SHAPES_E(X) NTT_SHAPES_F(X)
 
 #define NTT_SHAPE_INSTANCE_EXTERN(LR, LC)                                                                       \
     extern template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, false, LR, LC>(                    \
