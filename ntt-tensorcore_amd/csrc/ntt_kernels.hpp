@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace nttk {
 
 template <class V>
@@ -83,6 +85,7 @@ enum JobFlags : uint32_t {
     JF_MFMA = 32u,        // pass runs on the matrix cores (ntt_mfma.hpp)
     JF_POST4 = 64u,       // post == POST_FOURSTEP (selects the kernel instantiation)
     JF_POST_LDS = 128u,   // stage the post-multiply tables in LDS
+    JF_CT_CYC = 256u,     // CT table is cyclic (Z[2^l] = 1): round 0 skips products by 1
 };
 
 struct PassJob {
@@ -114,6 +117,45 @@ struct PassJob {
 // Threads of every pass-kernel workgroup (a compile-time constant so that the
 // uniform index arithmetic folds).
 constexpr uint32_t kNT = 512;
+
+// LDS addressing by byte offset.  The pass kernel declares no static LDS, so
+// its dynamic LDS starts at LDS address 0 (launch_pass_kernel checks the
+// kernel's static size once) and a byte offset is the address itself.  Built
+// this way the compiler folds constant parts of an address into the
+// instruction's offset field; through the extern array it adds the array's
+// (link-time) address to every address, one VALU op per access.
+template <class T>
+using lds_p = __attribute__((address_space(3))) T*;
+template <class T>
+__device__ __forceinline__ T lds_ld(uint32_t off) {
+    return *reinterpret_cast<lds_p<const T>>((size_t)off);
+}
+template <class T>
+__device__ __forceinline__ void lds_st(uint32_t off, T v) {
+    *reinterpret_cast<lds_p<T>>((size_t)off) = v;
+}
+
+// A table of Shoup pairs staged in LDS at byte offset `off` (a multiple of the
+// pair's size): indexes and offsets like the global pointer it replaces.  A
+// pair moves as one aligned 8- or 16-byte access.
+template <class V>
+struct LdsTw {
+    typedef uint64_t U64x2 __attribute__((ext_vector_type(2)));
+    using Raw = typename std::conditional<sizeof(V) == 4, uint64_t, U64x2>::type;  // TwP<V>'s bytes, aligned
+    uint32_t off;
+    __device__ __forceinline__ TwP<V> operator[](uint32_t i) const {
+        const Raw r = lds_ld<Raw>(off + i * (uint32_t)sizeof(TwP<V>));
+        if constexpr (sizeof(V) == 4) return TwP<V>{(V)r, (V)(r >> 32)};
+        else return TwP<V>{(V)r.x, (V)r.y};
+    }
+    __device__ __forceinline__ LdsTw operator+(uint32_t i) const { return LdsTw{off + i * (uint32_t)sizeof(TwP<V>)}; }
+    __device__ __forceinline__ void set(uint32_t i, TwP<V> t) const {
+        Raw r;
+        if constexpr (sizeof(V) == 4) r = (uint64_t)t.w | ((uint64_t)t.wp << 32);
+        else r = Raw{(uint64_t)t.w, (uint64_t)t.wp};
+        lds_st<Raw>(off + i * (uint32_t)sizeof(TwP<V>), r);
+    }
+};
 
 // XOR swizzle of the LDS word index: permutes the low 6 bits by higher bits so
 // that both contiguous runs and the bit-reversed read-out hit distinct banks.
@@ -150,19 +192,20 @@ __device__ __forceinline__ uint32_t bitrev_bits(uint32_t k, uint32_t bits) {
 // m register-resident CT levels [l0, l0+m) for one unit of 2^m elements.
 // slot0 = LDS byte offset of the unit's first element, vsl[v] = the byte XOR
 // offset of its v-th element; zb[lam] points at the level-(l0+lam) twiddles.
-template <int m, class M>
-__device__ __forceinline__ void ct_unit(unsigned char* lds, const TwP<typename M::V>* const zb[m], const M& mod,
-                                        uint32_t slot0, const uint32_t* vsl) {
+// ONE0: the first twiddle of every level, zb[lam][0], is 1 (round 0 of a
+// cyclic table: Z[2^l] = fac^(R >> (l+1)) with fac = 1), and those butterflies
+// skip the product: 2^m - 1 of the unit's m * 2^(m-1) products.
+template <int m, bool ONE0, class M, class ZP>
+__device__ __forceinline__ void ct_unit(const ZP* zb, const M& mod, uint32_t slot0, const uint32_t* vsl) {
     using V = typename M::V;
-    auto at = [&](uint32_t b) -> V& { return *reinterpret_cast<V*>(lds + b); };
     TwP<V> w[(1 << m) - 1];
 #pragma unroll
     for (int lam = 0; lam < m; lam++)
 #pragma unroll
-        for (int g = 0; g < (1 << lam); g++) w[(1 << lam) - 1 + g] = zb[lam][g];
+        for (int g = ONE0 ? 1 : 0; g < (1 << lam); g++) w[(1 << lam) - 1 + g] = zb[lam][g];
     V x[1 << m];
 #pragma unroll
-    for (int v = 0; v < (1 << m); v++) x[v] = at(slot0 ^ vsl[v]);
+    for (int v = 0; v < (1 << m); v++) x[v] = lds_ld<V>(slot0 ^ vsl[v]);
 #pragma unroll
     for (int lam = 0; lam < m; lam++) {
         const int half = 1 << (m - 1 - lam);
@@ -173,22 +216,22 @@ __device__ __forceinline__ void ct_unit(unsigned char* lds, const TwP<typename M
             for (int v0 = 0; v0 < half; v0++) {
                 const int v = g * 2 * half + v0;
                 const V a = x[v];
-                const V b = mod.mul(x[v + half], wt);
+                const V b = (ONE0 && g == 0) ? x[v + half] : mod.mul(x[v + half], wt);
                 x[v] = mod.add(a, b);
                 x[v + half] = mod.sub(a, b);
             }
         }
     }
 #pragma unroll
-    for (int v = 0; v < (1 << m); v++) at(slot0 ^ vsl[v]) = x[v];
+    for (int v = 0; v < (1 << m); v++) lds_st<V>(slot0 ^ vsl[v], x[v]);
 }
 
 // One round of m levels starting at level l0 over the whole tile.  A unit is
 // one column c and 2^m rows base + v*2^t, t = log_r - l0 - m, base = i0*2^(t+m)
 // + s (s < 2^t); unit index u = (i0 * 2^t + s) * C + c.
-template <int m, class M, class ZP>
-__device__ __forceinline__ void ct_round(typename M::V* lds, ZP z, const M& mod, uint32_t l0, uint32_t log_r,
-                                         uint32_t log_c) {
+// ONE0 (l0 = 0 only): the table is cyclic, see ct_unit.
+template <int m, bool ONE0, class M, class ZP>
+__device__ __forceinline__ void ct_round(ZP z, const M& mod, uint32_t l0, uint32_t log_r, uint32_t log_c) {
     const uint32_t log_t = log_r - l0 - m;
     const uint32_t units = 1u << (log_r + log_c - m);
     constexpr uint32_t VB = sizeof(typename M::V);
@@ -200,26 +243,37 @@ __device__ __forceinline__ void ct_round(typename M::V* lds, ZP z, const M& mod,
         // row-major word of (base, c): u's low log_t + log_c bits stay, i0 moves up by m
         const uint32_t word = (u & lowmask) | ((u & ~lowmask) << m);
         const uint32_t i0 = u >> (log_t + log_c);
-        const TwP<typename M::V>* zb[m];
+        ZP zb[m];
 #pragma unroll
-        for (int lam = 0; lam < m; lam++) zb[lam] = &z[(1u << (l0 + lam)) + (i0 << lam)];
-        ct_unit<m, M>(reinterpret_cast<unsigned char*>(lds), zb, mod, swz(word) * VB, vsl);
+        for (int lam = 0; lam < m; lam++) zb[lam] = z + ((1u << (l0 + lam)) + (i0 << lam));
+        ct_unit<m, ONE0, M>(zb, mod, swz(word) * VB, vsl);
     }
 }
 
 // All log_r levels, in rounds of at most MAXM levels balanced so that no
-// round is much shorter than the others.
+// round is much shorter than the others.  cyc: the table is cyclic (fac = 1),
+// so round 0 skips its products by 1.
 template <int MAXM, class M, class ZP>
-__device__ __forceinline__ void ct_levels(typename M::V* lds, ZP z, const M& mod, uint32_t log_r, uint32_t log_c) {
+__device__ __forceinline__ void ct_levels(ZP z, const M& mod, uint32_t log_r, uint32_t log_c, bool cyc) {
     const uint32_t rounds = (log_r + MAXM - 1) / MAXM;
     uint32_t l0 = 0;
     for (uint32_t i = 0; i < rounds; i++) {
         const uint32_t left = log_r - l0, m = (left + (rounds - i) - 1) / (rounds - i);
-        switch (m) {
-            case 1: ct_round<1>(lds, z, mod, l0, log_r, log_c); break;
-            case 2: ct_round<2>(lds, z, mod, l0, log_r, log_c); break;
-            case 3: ct_round<3>(lds, z, mod, l0, log_r, log_c); break;
-            default: ct_round<(MAXM >= 4 ? 4 : 3)>(lds, z, mod, l0, log_r, log_c); break;
+        constexpr int M4 = MAXM >= 4 ? 4 : 3;
+        if (i == 0 && cyc) {
+            switch (m) {
+                case 1: ct_round<1, true, M>(z, mod, l0, log_r, log_c); break;
+                case 2: ct_round<2, true, M>(z, mod, l0, log_r, log_c); break;
+                case 3: ct_round<3, true, M>(z, mod, l0, log_r, log_c); break;
+                default: ct_round<M4, true, M>(z, mod, l0, log_r, log_c); break;
+            }
+        } else {
+            switch (m) {
+                case 1: ct_round<1, false, M>(z, mod, l0, log_r, log_c); break;
+                case 2: ct_round<2, false, M>(z, mod, l0, log_r, log_c); break;
+                case 3: ct_round<3, false, M>(z, mod, l0, log_r, log_c); break;
+                default: ct_round<M4, false, M>(z, mod, l0, log_r, log_c); break;
+            }
         }
         l0 += m;
         lds_barrier();
@@ -229,13 +283,14 @@ __device__ __forceinline__ void ct_levels(typename M::V* lds, ZP z, const M& mod
 // ct_levels for a tile shape fixed at compile time: the rounds unrolled with
 // constant first levels, so every uniform offset is an immediate.
 template <int LR, int LC, int MAXM, int L0, class M, class ZP>
-__device__ __forceinline__ void ct_levels_c(typename M::V* lds, ZP z, const M& mod) {
+__device__ __forceinline__ void ct_levels_c(ZP z, const M& mod, bool cyc) {
     if constexpr (L0 < LR) {
         constexpr int rounds = (LR - L0 + MAXM - 1) / MAXM;
         constexpr int m = (LR - L0 + rounds - 1) / rounds;
-        ct_round<m>(lds, z, mod, (uint32_t)L0, (uint32_t)LR, (uint32_t)LC);
+        if (L0 == 0 && cyc) ct_round<m, true, M>(z, mod, (uint32_t)L0, (uint32_t)LR, (uint32_t)LC);
+        else ct_round<m, false, M>(z, mod, (uint32_t)L0, (uint32_t)LR, (uint32_t)LC);
         lds_barrier();
-        ct_levels_c<LR, LC, MAXM, L0 + m, M, ZP>(lds, z, mod);
+        ct_levels_c<LR, LC, MAXM, L0 + m, M, ZP>(z, mod, false);
     }
 }
 
@@ -277,10 +332,9 @@ constexpr int kUnroll = 8;
 //   SWT  tables are staged in LDS at tsw(i),
 //   LR, LC the tile shape log2 R, log2 C when fixed at compile time (LR = 0:
 //   taken from the job).
-template <int PER, int POST, int NTAB, int QM, bool PRED, bool SWT, int LR, int LC, class M, class Sout>
-__device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, unsigned char* lds_raw, uint32_t poly,
-                                           uint32_t col0, const TwP<typename M::V>* tl,
-                                           const TwP<typename M::V>* tm, const TwP<typename M::V>* th) {
+template <int PER, int POST, int NTAB, int QM, bool PRED, bool SWT, int LR, int LC, class M, class Sout, class TP>
+__device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint32_t poly, uint32_t col0, TP tl, TP tm,
+                                           TP th) {
     using V = typename M::V;
     constexpr uint32_t VB = sizeof(V);
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
@@ -288,7 +342,6 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, unsig
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
     const uint32_t T = kNT, tid = threadIdx.x;
-    auto lds_at = [&](uint32_t byte_off) -> V& { return *reinterpret_cast<V*>(lds_raw + byte_off); };
     const bool cf = J.flags & JF_STORE_CFAST;
     const uint32_t sweep = PER * T;
     const TwP<V> scale{(V)J.scale_w, (V)J.scale_wp};
@@ -340,7 +393,7 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, unsig
             const uint32_t s = l_slot ^ slot(uk, uc);
             V x[PER];
 #pragma unroll
-            for (int e = 0; e < PER; e++) x[e] = (!PRED || lane_ok) ? lds_at(s ^ e_x[e]) : (V)0;
+            for (int e = 0; e < PER; e++) x[e] = (!PRED || lane_ok) ? lds_ld<V>(s ^ e_x[e]) : (V)0;
             if (POST == 2) {
                 if (QM == 1) {
                     const uint32_t ex = expo(k, cg);
@@ -383,18 +436,17 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, unsig
     }
 }
 
-template <int PER, bool P4, bool SWT, int LR, int LC, class M, class Sout>
-__device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, unsigned char* lds_raw, uint32_t poly,
-                                               uint32_t col0, bool pred, const TwP<typename M::V>* tl,
-                                               const TwP<typename M::V>* tm, const TwP<typename M::V>* th) {
+template <int PER, bool P4, bool SWT, int LR, int LC, class M, class Sout, class TP>
+__device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, uint32_t poly, uint32_t col0, bool pred,
+                                               TP tl, TP tm, TP th) {
     const uint32_t post = J.post;
     if (P4 && post == POST_FOURSTEP) {
         const bool cf = J.flags & JF_STORE_CFAST;
         const int qm = PER == 4 ? (cf ? (J.e_shift >= 2 ? 1 : 0) : 2) : 0;
         const bool three = J.mid_bits != 0;
 #define NTT_STORE4(NT, QM)                                                                          \
-    (pred ? store_loop<PER, 2, NT, QM, true, SWT, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th) \
-          : store_loop<PER, 2, NT, QM, false, SWT, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th))
+    (pred ? store_loop<PER, 2, NT, QM, true, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th) \
+          : store_loop<PER, 2, NT, QM, false, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th))
         if (three) {
             if (qm == 1) NTT_STORE4(3, 1);
             else if (qm == 2) NTT_STORE4(3, 2);
@@ -406,25 +458,24 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
         }
 #undef NTT_STORE4
     } else if (post == POST_SCALE) {
-        if (pred) store_loop<PER, 1, 2, 0, true, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
-        else store_loop<PER, 1, 2, 0, false, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
+        if (pred) store_loop<PER, 1, 2, 0, true, false, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
+        else store_loop<PER, 1, 2, 0, false, false, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
     } else {
-        if (pred) store_loop<PER, 0, 2, 0, true, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
-        else store_loop<PER, 0, 2, 0, false, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, tl, tm, th);
+        if (pred) store_loop<PER, 0, 2, 0, true, false, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
+        else store_loop<PER, 0, 2, 0, false, false, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
     }
 }
 
 // Tile load into LDS (row-major, swizzled): PER elements per thread and
 // sweep, PRED per-element bounds predicates, LR/LC as in store_loop.
 template <int PER, bool PRED, int LR, int LC, class Sin, class V>
-__device__ __forceinline__ void load_loop(const PassJob& J, unsigned char* lds_raw, uint32_t poly, uint32_t col0) {
+__device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint32_t col0) {
     constexpr uint32_t VB = sizeof(V);
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
     const uint32_t R = 1u << log_r, C = 1u << log_c;
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
     const uint32_t T = kNT, tid = threadIdx.x;
-    auto lds_at = [&](uint32_t byte_off) -> V& { return *reinterpret_cast<V*>(lds_raw + byte_off); };
     const bool cf = J.flags & JF_LOAD_CFAST;
     const uint32_t sweep = PER * T;
     const uint32_t sh_sr = J.sh_sr, sh_sc = J.sh_sc;
@@ -474,15 +525,14 @@ __device__ __forceinline__ void load_loop(const PassJob& J, unsigned char* lds_r
             rc(ui, ur, uc);
             const uint32_t s = l_slot ^ slot(ur, uc);
 #pragma unroll
-            for (int e = 0; e < PER; e++) lds_at(s ^ e_x[e]) = (V)v[u][e];
+            for (int e = 0; e < PER; e++) lds_st<V>(s ^ e_x[e], (V)v[u][e]);
         }
     }
 }
 
 template <class M, class Sin, class Sout, int MAXM, bool P4, int LR, int LC>
-__device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job, unsigned char* lds_raw) {
+__device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job) {
     using V = typename M::V;
-    V* lds = reinterpret_cast<V*>(lds_raw);
     const uint32_t poly = tile_in_job / J.tiles_per_poly;
     const uint32_t tile = tile_in_job - poly * J.tiles_per_poly;
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
@@ -495,49 +545,50 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job,
     const M mod{(decltype(M::q))J.q};
     const uint32_t flags = J.flags;
 
-    // Tables staged in LDS behind the tile: the CT table (R entries), then the
-    // post-multiply tables (lo, mid, hi).
-    unsigned char* after = lds_raw + (size_t)total * sizeof(V);
-    TwP<V>* zl = reinterpret_cast<TwP<V>*>(after);
+    // LDS: the tile at offset 0, then the tables staged behind it: the CT
+    // table (R entries), then the post-multiply tables (lo, mid, hi).
+    uint32_t after = total * (uint32_t)sizeof(V);
+    const LdsTw<V> zl{after};
     const TwP<V>* zg = reinterpret_cast<const TwP<V>*>(J.z);
     if (flags & JF_Z_LDS) {
-        for (uint32_t i = tid; i < R; i += T) zl[i] = zg[i];
-        after += (size_t)R * sizeof(TwP<V>);
+        for (uint32_t i = tid; i < R; i += T) zl.set(i, zg[i]);
+        after += R * (uint32_t)sizeof(TwP<V>);
     }
     const uint32_t n_lo = 1u << J.lo_bits, n_mid = J.mid_bits ? 1u << J.mid_bits : 0u;
     const uint32_t n_hi = (J.e_mask >> (J.lo_bits + J.mid_bits)) + 1u;
-    TwP<V>* pl = reinterpret_cast<TwP<V>*>(after);
-    TwP<V>* pm = pl + n_lo;
-    TwP<V>* ph = pm + n_mid;
+    const LdsTw<V> pl{after};
+    const LdsTw<V> pm = pl + n_lo;
+    const LdsTw<V> ph = pm + n_mid;
     if (P4 && (flags & JF_POST_LDS)) {
         const TwP<V>* gl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
         const TwP<V>* gm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
         const TwP<V>* gh = reinterpret_cast<const TwP<V>*>(J.tw_hi);
-        for (uint32_t i = tid; i < n_lo; i += T) pl[tsw(i)] = gl[i];
-        for (uint32_t i = tid; i < n_mid; i += T) pm[tsw(i)] = gm[i];
-        for (uint32_t i = tid; i < n_hi; i += T) ph[tsw(i)] = gh[i];
+        for (uint32_t i = tid; i < n_lo; i += T) pl.set(tsw(i), gl[i]);
+        for (uint32_t i = tid; i < n_mid; i += T) pm.set(tsw(i), gm[i]);
+        for (uint32_t i = tid; i < n_hi; i += T) ph.set(tsw(i), gh[i]);
     }
 
     // ---- load ----
     // Predicates are needed for partial tiles and for tiles smaller than one sweep.
     const bool pred_l = !full || total < ((flags & JF_VEC4) ? 4u : 1u) * T;
     if (LR) {  // shape-specialised launches are full tiles with vector loads
-        load_loop<4, false, LR, LC, Sin, V>(J, lds_raw, poly, col0);
+        load_loop<4, false, LR, LC, Sin, V>(J, poly, col0);
     } else if (flags & JF_VEC4) {
-        if (pred_l) load_loop<4, true, LR, LC, Sin, V>(J, lds_raw, poly, col0);
-        else load_loop<4, false, LR, LC, Sin, V>(J, lds_raw, poly, col0);
+        if (pred_l) load_loop<4, true, LR, LC, Sin, V>(J, poly, col0);
+        else load_loop<4, false, LR, LC, Sin, V>(J, poly, col0);
     } else {
-        if (pred_l) load_loop<1, true, LR, LC, Sin, V>(J, lds_raw, poly, col0);
-        else load_loop<1, false, LR, LC, Sin, V>(J, lds_raw, poly, col0);
+        if (pred_l) load_loop<1, true, LR, LC, Sin, V>(J, poly, col0);
+        else load_loop<1, false, LR, LC, Sin, V>(J, poly, col0);
     }
     lds_barrier();
 
+    const bool cyc = flags & JF_CT_CYC;
     if (LR) {
-        if (flags & JF_Z_LDS) ct_levels_c<LR, LC, MAXM, 0, M, const TwP<V>*>(lds, (const TwP<V>*)zl, mod);
-        else ct_levels_c<LR, LC, MAXM, 0, M, const TwP<V>*>(lds, zg, mod);
+        if (flags & JF_Z_LDS) ct_levels_c<LR, LC, MAXM, 0, M>(zl, mod, cyc);
+        else ct_levels_c<LR, LC, MAXM, 0, M>(zg, mod, cyc);
     } else {
-        if (flags & JF_Z_LDS) ct_levels<MAXM>(lds, (const TwP<V>*)zl, mod, log_r, log_c);
-        else ct_levels<MAXM>(lds, zg, mod, log_r, log_c);
+        if (flags & JF_Z_LDS) ct_levels<MAXM, M>(zl, mod, log_r, log_c, cyc);
+        else ct_levels<MAXM, M>(zg, mod, log_r, log_c, cyc);
     }
 
     const bool pred_s = !full || total < ((flags & JF_VEC4_ST) ? 4u : 1u) * T;
@@ -545,13 +596,13 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job,
     const TwP<V>* tm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
     const TwP<V>* th = reinterpret_cast<const TwP<V>*>(J.tw_hi);
     if (LR) {  // shape-specialised: vector stores, full tiles, LDS post tables
-        store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, false, pl, pm, ph);
+        store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, false, pl, pm, ph);
     } else if (P4 && (flags & JF_POST_LDS)) {
-        if (flags & JF_VEC4_ST) store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, pl, pm, ph);
-        else store_dispatch<1, P4, true, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, pl, pm, ph);
+        if (flags & JF_VEC4_ST) store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, pl, pm, ph);
+        else store_dispatch<1, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, pl, pm, ph);
     } else {
-        if (flags & JF_VEC4_ST) store_dispatch<4, P4, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, tl, tm, th);
-        else store_dispatch<1, P4, false, LR, LC, M, Sout>(J, mod, lds_raw, poly, col0, pred_s, tl, tm, th);
+        if (flags & JF_VEC4_ST) store_dispatch<4, P4, false, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, tl, tm, th);
+        else store_dispatch<1, P4, false, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, tl, tm, th);
     }
 }
 
@@ -582,14 +633,15 @@ __device__ __forceinline__ uint32_t find_job(const PassJob* __restrict__ jobs, u
 // stores, full tiles and (with P4) LDS post tables; see pass_shape_ok().
 template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC>
 __global__ void __launch_bounds__(NT) pass_kernel(const PassJob* __restrict__ jobs, uint32_t njobs, PassJob inl) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
+    // The dynamic LDS (no static LDS: it starts at address 0) is addressed by
+    // byte offset through lds_ld / lds_st.
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     if (njobs == 0) {
-        run_pass<M, Sin, Sout, MAXM, P4, LR, LC>(inl, tile, lds_raw);
+        run_pass<M, Sin, Sout, MAXM, P4, LR, LC>(inl, tile);
         return;
     }
     const PassJob J = jobs[find_job(jobs, njobs, tile)];
-    run_pass<M, Sin, Sout, MAXM, P4, LR, LC>(J, tile - J.tile_begin, lds_raw);
+    run_pass<M, Sin, Sout, MAXM, P4, LR, LC>(J, tile - J.tile_begin);
 }
 
 // ---- small elementwise kernels ----

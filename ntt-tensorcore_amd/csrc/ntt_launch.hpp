@@ -20,9 +20,16 @@ hipError_t launch_pass_kernel(const PassJob* djobs, uint32_t njobs, const PassJo
                               hipStream_t stream) {
     auto fn = pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC>;
     static std::once_flag once;
+    static hipError_t ready = hipSuccess;
     std::call_once(once, [&] {
         (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPassLdsMax);
+        // The kernel addresses its dynamic LDS from address 0 (lds_ld/lds_st),
+        // which holds only while it has no static LDS.
+        hipFuncAttributes at{};
+        ready = hipFuncGetAttributes(&at, (const void*)fn);
+        if (ready == hipSuccess && at.sharedSizeBytes != 0) ready = hipErrorInvalidKernelFile;
     });
+    if (ready != hipSuccess) return ready;
     hipLaunchKernelGGL(fn, dim3(ntiles), dim3(kPassThreads), lds, stream, djobs, njobs, inl);
     return hipGetLastError();
 }

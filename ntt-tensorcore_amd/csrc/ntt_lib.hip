@@ -172,6 +172,7 @@ static Split choose_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight) {
 struct PassSpec {
     uint32_t log_r = 0, log_c = 0, log_l = 0;  // rows, tile columns, product of earlier radices
     bool mfma = false;                         // z_off then points at an MFMA table block
+    bool cyc = false;                          // CT table without the negacyclic factor (Z[2^l] = 1)
     int z_off = -1;
     uint32_t post = POST_NONE;
     int lo_off = -1, hi_off = -1, mid_off = -1;
@@ -368,6 +369,7 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
         s.log_l = log_l;
         s.log_c = std::min(sp.log_e > lr ? sp.log_e - lr : 0u, log_n - lr);
         const uint64_t fac = (i == 0 && nega && !inv) ? powmod(p, N >> lr, q) : 1;
+        s.cyc = fac == 1;
         // MFMA passes: R = 4096 any width; R = 64 needs >= 16 columns; never single-pass
         s.mfma = ((sp.mfma >> i) & 1u) && mfma_ok(lr, arith) && sp.npass >= 2 && (lr == 12 || log_n - lr >= 4);
         if (s.mfma) {
@@ -569,6 +571,7 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         }
         set_shifts(J);
         finish_flags(J, P.arith, src_st, dst_st, s.mfma);
+        if (s.cyc && !s.mfma) J.flags |= JF_CT_CYC;
         src = dst;
         src_st = dst_st;
     }
