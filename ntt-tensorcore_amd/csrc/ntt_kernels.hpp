@@ -75,7 +75,9 @@ enum PostMode : uint32_t { POST_NONE = 0, POST_FOURSTEP = 2, POST_SCALE = 3 };
 // store of the first pass and the grouped store of middle passes.
 //   post-multiply exponent  e = (e_a*k*u + e_b*k + e_d*u) mod (e_mask+1),
 //   u = c >> e_shift, factor = lo[e mod 2^lo_bits] * mid[(e >> lo_bits) mod
-//   2^mid_bits] * hi[e >> (lo_bits + mid_bits)] (no mid factor if mid_bits = 0).
+//   2^mid_bits] * hi[e >> (lo_bits + mid_bits)] (no mid factor if mid_bits = 0),
+//   or, when e is constant across a tile's columns (JF_POST_ROW), the row
+//   table entry of (u, k) staged for the tile.
 enum JobFlags : uint32_t {
     JF_LOAD_CFAST = 1u,   // consecutive threads walk columns (s_c == 1), else rows (s_r == 1)
     JF_STORE_CFAST = 2u,  // consecutive threads walk columns (o_c == 1), else rows (o_r == 1)
@@ -86,6 +88,7 @@ enum JobFlags : uint32_t {
     JF_POST4 = 64u,       // post == POST_FOURSTEP (selects the kernel instantiation)
     JF_POST_LDS = 128u,   // stage the post-multiply tables in LDS
     JF_CT_CYC = 256u,     // CT table is cyclic (Z[2^l] = 1): round 0 skips products by 1
+    JF_POST_ROW = 512u,   // post from the row table (one factor per row k), staged in LDS
 };
 
 struct PassJob {
@@ -95,6 +98,7 @@ struct PassJob {
     const void* tw_lo;    // post-multiply tables (TwP<V>)
     const void* tw_hi;
     const void* tw_mid;   // middle table of a three-table split (mid_bits > 0)
+    const void* tw_row;   // row table: factor of (u, k) at [u*R + k], u = first column >> row_shift
     uint64_t q;
     uint64_t scale_w, scale_wp;  // POST_SCALE constant (Shoup pair)
     int64_t s_r, s_c;
@@ -108,6 +112,7 @@ struct PassJob {
     uint32_t tile_begin;         // first workgroup of this job in the launch
     uint32_t npoly;
     uint32_t flags;
+    uint32_t row_shift;
     // log2 of s_r, s_c, o_r, o_c, o_hs (all powers of two; a zero stride only
     // ever multiplies a zero term, so its shift is arbitrary)
     uint8_t sh_sr, sh_sc, sh_or, sh_oc;
@@ -327,7 +332,8 @@ constexpr int kUnroll = 8;
 //   PER  elements per thread and sweep (4: vector store, 1: scalar),
 //   POST 0: none, 1: constant (POST_SCALE), 2: power table (POST_FOURSTEP),
 //   NTAB tables of the power (2 or 3), QM exponent scheme within a quad
-//   (0: per element, 1: one exponent per quad, 2: a fixed step per element),
+//   (0: per element, 1: one exponent per quad, 2: a fixed step per element,
+//   3: row table, the factor of row k at tl[k]),
 //   PRED per-element bounds predicates (partial tiles, tiles smaller than a sweep),
 //   SWT  tables are staged in LDS at tsw(i),
 //   LR, LC the tile shape log2 R, log2 C when fixed at compile time (LR = 0:
@@ -395,7 +401,16 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint3
 #pragma unroll
             for (int e = 0; e < PER; e++) x[e] = (!PRED || lane_ok) ? lds_ld<V>(s ^ e_x[e]) : (V)0;
             if (POST == 2) {
-                if (QM == 1) {
+                if (QM == 3) {  // row table staged for this tile: entry k
+                    if (cf) {
+                        const TwP<V> f = tl[k];
+#pragma unroll
+                        for (int e = 0; e < PER; e++) x[e] = mod.mul(x[e], f);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < PER; e++) x[e] = mod.mul(x[e], tl[k + (uint32_t)e]);
+                    }
+                } else if (QM == 1) {
                     const uint32_t ex = expo(k, cg);
                     const TwP<V> f0 = tl[ti(ex & m0)], f2 = th[ti(ex >> b01)];
                     TwP<V> f1{};
@@ -440,7 +455,10 @@ template <int PER, bool P4, bool SWT, int LR, int LC, class M, class Sout, class
 __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, uint32_t poly, uint32_t col0, bool pred,
                                                TP tl, TP tm, TP th) {
     const uint32_t post = J.post;
-    if (P4 && post == POST_FOURSTEP) {
+    if (P4 && post == POST_FOURSTEP && SWT && (J.flags & JF_POST_ROW)) {
+        if (pred) store_loop<PER, 2, 2, 3, true, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
+        else store_loop<PER, 2, 2, 3, false, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
+    } else if (P4 && post == POST_FOURSTEP) {
         const bool cf = J.flags & JF_STORE_CFAST;
         const int qm = PER == 4 ? (cf ? (J.e_shift >= 2 ? 1 : 0) : 2) : 0;
         const bool three = J.mid_bits != 0;
@@ -559,7 +577,10 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
     const LdsTw<V> pl{after};
     const LdsTw<V> pm = pl + n_lo;
     const LdsTw<V> ph = pm + n_mid;
-    if (P4 && (flags & JF_POST_LDS)) {
+    if (P4 && (flags & JF_POST_ROW)) {
+        const TwP<V>* gr = reinterpret_cast<const TwP<V>*>(J.tw_row) + ((size_t)(col0 >> J.row_shift) << log_r);
+        for (uint32_t i = tid; i < R; i += T) pl.set(i, gr[i]);
+    } else if (P4 && (flags & JF_POST_LDS)) {
         const TwP<V>* gl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
         const TwP<V>* gm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
         const TwP<V>* gh = reinterpret_cast<const TwP<V>*>(J.tw_hi);

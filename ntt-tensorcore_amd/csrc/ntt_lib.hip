@@ -176,6 +176,8 @@ struct PassSpec {
     int z_off = -1;
     uint32_t post = POST_NONE;
     int lo_off = -1, hi_off = -1, mid_off = -1;
+    int row_off = -1;        // row table of the post (see TableBuilder::rows), -1 if none
+    uint32_t row_shift = 0;  // tile row index u = first column >> row_shift
     uint32_t e_a = 0, e_b = 0, e_d = 0, e_shift = 0, e_mask = 0, lo_bits = 0, mid_bits = 0;
     uint64_t scale_w = 0, scale_wp = 0;
 };
@@ -256,7 +258,27 @@ struct TableBuilder {
         s.mid_off = add(powers(powmod(base, 1ull << b0, q), 1ull << b1, q));
         s.hi_off = add(powers(powmod(base, 1ull << (b0 + b1), q), 1ull << b2, q, mult));
     }
+    // Row table of a post whose exponent is constant across a tile's columns
+    // (u = c >> e_shift fixed per tile, or no u term): entry u*R + k is
+    // base^e(k, u) * mult, so the store multiplies by one staged entry per
+    // row.  base has order dividing 2^mbits, so the exponent needs no mask.
+    void rows(PassSpec& s, uint64_t base, uint64_t mult, uint32_t R, uint32_t U, uint32_t row_shift) {
+        std::vector<uint64_t> v((size_t)U * R);
+        for (uint32_t u = 0; u < U; u++) {
+            uint64_t f = mulmod(powmod(base, (uint64_t)s.e_d * u, q), mult, q);
+            const uint64_t st = powmod(base, (uint64_t)s.e_a * u + s.e_b, q);
+            for (uint32_t k = 0; k < R; k++) {
+                v[(size_t)u * R + k] = f;
+                f = mulmod(f, st, q);
+            }
+        }
+        s.row_off = add(v);
+        s.row_shift = row_shift;
+    }
 };
+
+// Row-table size limit (entries): larger tables keep the power-table post.
+static const uint64_t kRowTableMax = 1ull << 20;
 
 // Balanced base-256 digits of v (|v| < 2^30): v = sum_i d_i 256^i, d_i in [-128, 127].
 static void balanced_digits(int64_t v, int8_t d[4]) {
@@ -361,6 +383,18 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
     const uint64_t ninv = invmod(N % q, q);
     const int sbits = arith == Arith::M32 ? 32 : 64;
     TableBuilder tb{arith, q, {}};
+    // Post-multiply tables of pass s: the power tables, plus a row table when
+    // the exponent is constant across a tile's columns (no u term, or u =
+    // c >> e_shift with C <= 2^e_shift) and the table is small.
+    auto post = [&](PassSpec& s, uint64_t base, uint32_t mbits, uint64_t mult) {
+        tb.pow_tables(s, base, mbits, mult);
+        if (s.mfma) return;
+        const bool u_free = s.e_a == 0 && s.e_d == 0;
+        if (!u_free && s.e_shift < s.log_c) return;
+        const uint32_t R = 1u << s.log_r;
+        const uint64_t U = u_free ? 1 : std::max<uint64_t>(1, (N >> s.log_r) >> s.e_shift);
+        if (U * R <= kRowTableMax) tb.rows(s, base, mult, R, (uint32_t)U, u_free ? 31u : s.e_shift);
+    };
     uint32_t log_l = 0;
     for (int i = 0; i < P->npass; i++) {
         PassSpec& s = P->pass[i];
@@ -384,25 +418,25 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
         if (i < P->npass - 1) {
             s.post = POST_FOURSTEP;
             if (i == 0) {
-                if (!nega) { s.e_a = 1; tb.pow_tables(s, w, log_n, inv ? ninv : 1); }
-                else if (!inv) { s.e_a = 2; s.e_d = 1; tb.pow_tables(s, p, log_n + 1, 1); }
-                else { s.e_a = 2; s.e_b = 1; tb.pow_tables(s, p, log_n + 1, ninv); }
+                if (!nega) { s.e_a = 1; post(s, w, log_n, inv ? ninv : 1); }
+                else if (!inv) { s.e_a = 2; s.e_d = 1; post(s, p, log_n + 1, 1); }
+                else { s.e_a = 2; s.e_b = 1; post(s, p, log_n + 1, ninv); }
             } else {
                 s.e_a = 1;
                 s.e_shift = log_l;
-                tb.pow_tables(s, powmod(w, 1ull << log_l, q), log_n - log_l, 1);
+                post(s, powmod(w, 1ull << log_l, q), log_n - log_l, 1);
             }
         } else if (inv && nega) {
             s.post = POST_FOURSTEP;
             if (P->npass == 1) {
                 s.e_b = 1;
-                tb.pow_tables(s, p, log_n + 1, ninv);
+                post(s, p, log_n + 1, ninv);
             } else {
                 const uint32_t lr0 = sp.log_r[0];
                 s.e_b = 1u << (log_l - lr0);
                 s.e_d = 1;
                 s.e_shift = lr0;
-                tb.pow_tables(s, powmod(p, 1ull << lr0, q), log_n - lr0 + 1, 1);
+                post(s, powmod(p, 1ull << lr0, q), log_n - lr0 + 1, 1);
             }
         } else if (inv && P->npass == 1) {
             s.post = POST_SCALE;
@@ -470,8 +504,11 @@ static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma)
     uint32_t used = tile;
     if (J.post == POST_FOURSTEP) {
         f |= JF_POST4;
-        const uint32_t pb = post_table_entries(J) * 2 * vbytes(ar);
-        if (used + pb <= kLdsMax) {
+        const uint32_t rb = R * 2 * vbytes(ar), pb = post_table_entries(J) * 2 * vbytes(ar);
+        if (J.tw_row && used + rb <= kLdsMax) {
+            f |= JF_POST_LDS | JF_POST_ROW;
+            used += rb;
+        } else if (used + pb <= kLdsMax) {
             f |= JF_POST_LDS;
             used += pb;
         }
@@ -488,7 +525,8 @@ static uint32_t job_lds(const PassJob& J, Arith ar) {
     }
     uint32_t b = (R << J.log_c) * vbytes(ar);
     if (J.flags & JF_Z_LDS) b += R * 2 * vbytes(ar);
-    if (J.flags & JF_POST_LDS) b += post_table_entries(J) * 2 * vbytes(ar);
+    if (J.flags & JF_POST_ROW) b += R * 2 * vbytes(ar);
+    else if (J.flags & JF_POST_LDS) b += post_table_entries(J) * 2 * vbytes(ar);
     return b;
 }
 
@@ -534,6 +572,8 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         J.tw_lo = s.lo_off >= 0 ? blk + s.lo_off : nullptr;
         J.tw_hi = s.hi_off >= 0 ? blk + s.hi_off : nullptr;
         J.tw_mid = s.mid_off >= 0 ? blk + s.mid_off : nullptr;
+        J.tw_row = s.row_off >= 0 ? blk + s.row_off : nullptr;
+        J.row_shift = s.row_shift;
         J.q = P.q;
         J.scale_w = s.scale_w;
         J.scale_wp = s.scale_wp;
