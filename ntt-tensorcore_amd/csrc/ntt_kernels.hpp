@@ -51,6 +51,24 @@ struct Mod32 {
     }
 };
 
+// Montgomery product x * F * 2^-32 mod q (q < 2^31 odd, x, F < q), canonical.
+// m = (x*F mod 2^32) * qinv makes m*q agree with x*F in the low word, so
+// x*F - m*q = (hi(x*F) - hi(m*q)) * 2^32 exactly, and that difference lies in
+// (-q, q): one conditional add of q (the min trick of Mod32::sub).
+__device__ __forceinline__ uint32_t mont32(uint32_t x, uint32_t F, uint32_t q, uint32_t qinv) {
+    const uint32_t lo = x * F, hi = __umulhi(x, F);
+    const uint32_t r = hi - __umulhi(lo * qinv, q);
+    return min(r, r + q);
+}
+
+// q^-1 mod 2^32 for odd q: Newton's iteration from q (right to 3 bits).
+__device__ __forceinline__ uint32_t inv_mod_2_32(uint32_t q) {
+    uint32_t x = q;
+#pragma unroll
+    for (int i = 0; i < 4; i++) x *= 2u - q * x;
+    return x;
+}
+
 // q < 2^62: 2q < 2^63, Shoup with a 64-bit quotient.
 struct Mod64 {
     using V = uint64_t;
@@ -89,6 +107,7 @@ enum JobFlags : uint32_t {
     JF_POST_LDS = 128u,   // stage the post-multiply tables in LDS
     JF_CT_CYC = 256u,     // CT table is cyclic (Z[2^l] = 1): round 0 skips products by 1
     JF_POST_ROW = 512u,   // post from the row table (one factor per row k), staged in LDS
+    JF_POST_STEP = 1024u, // row-fast first-pass store with per-column stepped factors (tw_col)
 };
 
 struct PassJob {
@@ -99,8 +118,9 @@ struct PassJob {
     const void* tw_hi;
     const void* tw_mid;   // middle table of a three-table split (mid_bits > 0)
     const void* tw_row;   // row table: factor of (u, k) at [u*R + k], u = first column >> row_shift
+    const void* tw_col;   // column table of JF_POST_STEP: 4 Shoup pairs per column
     uint64_t q;
-    uint64_t scale_w, scale_wp;  // POST_SCALE constant (Shoup pair)
+    uint64_t scale_w, scale_wp;  // POST_SCALE constant; with JF_POST_STEP, 2^32 mod q (Shoup pair)
     int64_t s_r, s_c;
     int64_t o_r, o_c, o_hs;
     int64_t in_pstride, out_pstride;
@@ -451,6 +471,65 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint3
     }
 }
 
+// Store of a first pass (row-fast output, 4 consecutive rows k per vector
+// store) whose post exponent is linear in k for each column:
+// e(k, c) = s(c)*k + e_d*c with s(c) = e_a*c + e_b.  Each thread keeps one
+// column and walks down its rows, so the factor steps by constants of that
+// column instead of being looked up per element: thread t takes column
+// t >> log_tpc and rows 4m + kstep*u + j (m = t mod 2^log_tpc,
+// kstep = 4 * 2^log_tpc, j < 4).  G, the factor of row 4m + kstep*u in
+// Montgomery form, starts from the power tables and steps by
+// Wb = base^(s*kstep); rows +1..+3 first take base^(s*j) (Shoup), then every
+// element one Montgomery product by G.  Column table tw_col[c] = {base^s,
+// base^2s, base^3s, base^(s*kstep)}; 32-bit arithmetic, full tiles.
+template <int NTAB, bool SWT, int LR, int LC, class Sout, class TP>
+__device__ __forceinline__ void store_loop_step(const PassJob& J, const Mod32& mod, uint32_t poly, uint32_t col0,
+                                                TP tl, TP tm, TP th) {
+    using V = uint32_t;
+    const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t log_tpc = 9u - log_c;  // kNT = 2^9 threads, 2^log_tpc per column
+    const uint32_t cl = tid >> log_tpc, m = tid & ((1u << log_tpc) - 1u);
+    const uint32_t cg = col0 + cl;
+    const uint32_t kstep = 4u << log_tpc;
+    const uint32_t q = mod.q, qinv = inv_mod_2_32(q);
+    const TwP<V>* ct = reinterpret_cast<const TwP<V>*>(J.tw_col) + (size_t)cg * 4u;
+    const TwP<V> B1 = ct[0], B2 = ct[1], B3 = ct[2], Wb = ct[3];
+    // G = base^(e_d*c + s*4m) * mult * 2^32 mod q
+    const uint32_t b0 = J.lo_bits, b1 = J.mid_bits, b01 = b0 + b1;
+    const uint32_t e0 = (J.e_d * cg + (J.e_a * cg + J.e_b) * (4u * m)) & J.e_mask;
+    auto ti = [&](uint32_t i) -> uint32_t { return SWT ? tsw(i) : i; };
+    V G = tl[ti(e0 & ((1u << b0) - 1u))].w;
+    if (NTAB == 3) G = mod.mul(G, tm[ti((e0 >> b0) & ((1u << b1) - 1u))]);
+    G = mod.mul(G, th[ti(e0 >> b01)]);
+    G = mod.mul(G, TwP<V>{(V)J.scale_w, (V)J.scale_wp});
+    // LDS byte offset of (row k, column c) is linear in the bit fields of k
+    auto slot = [&](uint32_t k, uint32_t c) -> uint32_t { return swz((bitrev_bits(k, log_r) << log_c) | c) * 4u; };
+    const uint32_t l_slot = slot(4u * m, cl);
+    uint32_t jx[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) jx[j] = slot((uint32_t)j, 0);
+    const uint32_t o_lbits = J.o_lbits, omask = (1u << o_lbits) - 1u;
+    Sout* __restrict__ out = reinterpret_cast<Sout*>(J.out) + (int64_t)poly * J.out_pstride +
+                             ((int64_t)(col0 >> o_lbits) << J.sh_ohs) + ((int64_t)(col0 & omask) << J.sh_oc) +
+                             (((cl >> o_lbits) << J.sh_ohs) + ((cl & omask) << J.sh_oc) + ((4u * m) << J.sh_or));
+    const uint32_t sweeps = 1u << (log_r - 2u - log_tpc);
+    for (uint32_t u = 0; u < sweeps; u++) {
+        const uint32_t su = l_slot ^ slot(u * kstep, 0);
+        V x[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) x[j] = lds_ld<V>(su ^ jx[j]);
+        x[1] = mod.mul(x[1], B1);
+        x[2] = mod.mul(x[2], B2);
+        x[3] = mod.mul(x[3], B3);
+#pragma unroll
+        for (int j = 0; j < 4; j++) x[j] = mont32(x[j], G, q, qinv);
+        G = mod.mul(G, Wb);
+        Sout v[4] = {(Sout)x[0], (Sout)x[1], (Sout)x[2], (Sout)x[3]};
+        store4<Sout>(out + ((u * kstep) << J.sh_or), v);
+    }
+}
+
 template <int PER, bool P4, bool SWT, int LR, int LC, class M, class Sout, class TP>
 __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, uint32_t poly, uint32_t col0, bool pred,
                                                TP tl, TP tm, TP th) {
@@ -458,6 +537,12 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
     if (P4 && post == POST_FOURSTEP && SWT && (J.flags & JF_POST_ROW)) {
         if (pred) store_loop<PER, 2, 2, 3, true, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
         else store_loop<PER, 2, 2, 3, false, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
+    } else if (std::is_same<M, Mod32>::value && PER == 4 && P4 && post == POST_FOURSTEP && SWT && !pred &&
+               (J.flags & JF_POST_STEP)) {
+        if constexpr (std::is_same<M, Mod32>::value && PER == 4) {
+            if (J.mid_bits) store_loop_step<3, SWT, LR, LC, Sout>(J, mod, poly, col0, tl, tm, th);
+            else store_loop_step<2, SWT, LR, LC, Sout>(J, mod, poly, col0, tl, tm, th);
+        }
     } else if (P4 && post == POST_FOURSTEP) {
         const bool cf = J.flags & JF_STORE_CFAST;
         const int qm = PER == 4 ? (cf ? (J.e_shift >= 2 ? 1 : 0) : 2) : 0;

@@ -177,6 +177,7 @@ struct PassSpec {
     uint32_t post = POST_NONE;
     int lo_off = -1, hi_off = -1, mid_off = -1;
     int row_off = -1;        // row table of the post (see TableBuilder::rows), -1 if none
+    int col_off = -1;        // column table of a stepped first-pass post (TableBuilder::cols)
     uint32_t row_shift = 0;  // tile row index u = first column >> row_shift
     uint32_t e_a = 0, e_b = 0, e_d = 0, e_shift = 0, e_mask = 0, lo_bits = 0, mid_bits = 0;
     uint64_t scale_w = 0, scale_wp = 0;
@@ -274,6 +275,27 @@ struct TableBuilder {
         }
         s.row_off = add(v);
         s.row_shift = row_shift;
+    }
+    // Column table of a first-pass post with exponent s(c)*k + e_d*c,
+    // s(c) = e_a*c + e_b (the stepped store, store_loop_step): per column
+    // base^s, base^2s, base^3s, base^(s*kstep).  Also 2^32 mod q as the pass's
+    // scale constant (the Montgomery factor of the stepped product).
+    void cols(PassSpec& s, uint64_t base, uint32_t ncols, uint32_t kstep) {
+        std::vector<uint64_t> v((size_t)ncols * 4);
+        const uint64_t ga = powmod(base, s.e_a, q), gak = powmod(ga, kstep, q);
+        uint64_t b = powmod(base, s.e_b, q), bk = powmod(b, kstep, q);
+        for (uint32_t c = 0; c < ncols; c++) {
+            const uint64_t b2 = mulmod(b, b, q);
+            v[(size_t)c * 4 + 0] = b;
+            v[(size_t)c * 4 + 1] = b2;
+            v[(size_t)c * 4 + 2] = mulmod(b2, b, q);
+            v[(size_t)c * 4 + 3] = bk;
+            b = mulmod(b, ga, q);
+            bk = mulmod(bk, gak, q);
+        }
+        s.col_off = add(v);
+        s.scale_w = (1ull << 32) % q;
+        s.scale_wp = shoup(s.scale_w, q, 32);
     }
 };
 
@@ -386,12 +408,20 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
     // Post-multiply tables of pass s: the power tables, plus a row table when
     // the exponent is constant across a tile's columns (no u term, or u =
     // c >> e_shift with C <= 2^e_shift) and the table is small.
-    auto post = [&](PassSpec& s, uint64_t base, uint32_t mbits, uint64_t mult) {
+    // The first pass of a P-pass plan (row-fast store) with 32-bit arithmetic
+    // steps its factors down each column instead (column table), when every
+    // thread gets whole quads: log_c <= 9 and R >= 4 * 2^(9 - log_c).
+    auto post = [&](PassSpec& s, uint64_t base, uint32_t mbits, uint64_t mult, bool first) {
         tb.pow_tables(s, base, mbits, mult);
         if (s.mfma) return;
+        const uint32_t R = 1u << s.log_r;
+        if (first && P->npass >= 2 && arith == Arith::M32 && s.log_c <= 9) {
+            const uint32_t kstep = 4u << (9 - s.log_c);
+            if (R >= kstep && (N >> s.log_r) <= kRowTableMax) tb.cols(s, base, (uint32_t)(N >> s.log_r), kstep);
+            return;
+        }
         const bool u_free = s.e_a == 0 && s.e_d == 0;
         if (!u_free && s.e_shift < s.log_c) return;
-        const uint32_t R = 1u << s.log_r;
         const uint64_t U = u_free ? 1 : std::max<uint64_t>(1, (N >> s.log_r) >> s.e_shift);
         if (U * R <= kRowTableMax) tb.rows(s, base, mult, R, (uint32_t)U, u_free ? 31u : s.e_shift);
     };
@@ -418,25 +448,25 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
         if (i < P->npass - 1) {
             s.post = POST_FOURSTEP;
             if (i == 0) {
-                if (!nega) { s.e_a = 1; post(s, w, log_n, inv ? ninv : 1); }
-                else if (!inv) { s.e_a = 2; s.e_d = 1; post(s, p, log_n + 1, 1); }
-                else { s.e_a = 2; s.e_b = 1; post(s, p, log_n + 1, ninv); }
+                if (!nega) { s.e_a = 1; post(s, w, log_n, inv ? ninv : 1, true); }
+                else if (!inv) { s.e_a = 2; s.e_d = 1; post(s, p, log_n + 1, 1, true); }
+                else { s.e_a = 2; s.e_b = 1; post(s, p, log_n + 1, ninv, true); }
             } else {
                 s.e_a = 1;
                 s.e_shift = log_l;
-                post(s, powmod(w, 1ull << log_l, q), log_n - log_l, 1);
+                post(s, powmod(w, 1ull << log_l, q), log_n - log_l, 1, false);
             }
         } else if (inv && nega) {
             s.post = POST_FOURSTEP;
             if (P->npass == 1) {
                 s.e_b = 1;
-                post(s, p, log_n + 1, ninv);
+                post(s, p, log_n + 1, ninv, false);
             } else {
                 const uint32_t lr0 = sp.log_r[0];
                 s.e_b = 1u << (log_l - lr0);
                 s.e_d = 1;
                 s.e_shift = lr0;
-                post(s, powmod(p, 1ull << lr0, q), log_n - lr0 + 1, 1);
+                post(s, powmod(p, 1ull << lr0, q), log_n - lr0 + 1, 1, false);
             }
         } else if (inv && P->npass == 1) {
             s.post = POST_SCALE;
@@ -573,6 +603,7 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         J.tw_hi = s.hi_off >= 0 ? blk + s.hi_off : nullptr;
         J.tw_mid = s.mid_off >= 0 ? blk + s.mid_off : nullptr;
         J.tw_row = s.row_off >= 0 ? blk + s.row_off : nullptr;
+        J.tw_col = s.col_off >= 0 ? blk + s.col_off : nullptr;
         J.row_shift = s.row_shift;
         J.q = P.q;
         J.scale_w = s.scale_w;
@@ -612,6 +643,10 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         set_shifts(J);
         finish_flags(J, P.arith, src_st, dst_st, s.mfma);
         if (s.cyc && !s.mfma) J.flags |= JF_CT_CYC;
+        // the stepped store needs staged power tables and row-fast vector stores
+        if (J.tw_col && (J.flags & JF_POST_LDS) && !(J.flags & (JF_POST_ROW | JF_STORE_CFAST)) &&
+            (J.flags & JF_VEC4_ST))
+            J.flags |= JF_POST_STEP;
         src = dst;
         src_st = dst_st;
     }
