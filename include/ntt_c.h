@@ -150,8 +150,9 @@ ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* desc, void* str
 
 /* One transform with an explicit radix split n = 2^log_r[0] * ... * 2^log_r[npass-1]
  * (npass <= 4), a tile of 2^log_e elements per workgroup, and bit i of
- * mfma_mask asking pass i to run on the INT8 matrix cores (honoured for
- * 2^log_r[i] = 64 or 4096 with q < 2^31).  No reference counterpart (the
+ * mfma_mask asking pass i to run its radix-16 rounds on the INT8 matrix cores
+ * (honoured for 2^log_r[i] >= 16, q < 2^31 and tiles of >= 2^8 elements;
+ * log_r[i] mod 4 levels stay on the vector ALUs).  No reference counterpart (the
  * reference fixes its split at radix-64 rounds, src/ntt_optimized.cu:529-554);
  * used by the autotuner and by tests that cover every plan shape. */
 ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc* desc, uint32_t npass, const uint32_t* log_r,

@@ -102,7 +102,7 @@ enum JobFlags : uint32_t {
     JF_VEC4 = 4u,         // 4 consecutive elements per thread on the load side (aligned)
     JF_Z_LDS = 8u,        // stage the CT table in LDS
     JF_VEC4_ST = 16u,     // 4 consecutive elements per thread on the store side
-    JF_MFMA = 32u,        // pass runs on the matrix cores (ntt_mfma.hpp)
+    JF_MFMA = 32u,        // radix-16 rounds on the matrix cores (ntt_mf16.hpp), q < 2^31
     JF_POST4 = 64u,       // post == POST_FOURSTEP (selects the kernel instantiation)
     JF_POST_LDS = 128u,   // stage the post-multiply tables in LDS
     JF_CT_CYC = 256u,     // CT table is cyclic (Z[2^l] = 1): round 0 skips products by 1
@@ -119,6 +119,7 @@ struct PassJob {
     const void* tw_mid;   // middle table of a three-table split (mid_bits > 0)
     const void* tw_row;   // row table: factor of (u, k) at [u*R + k], u = first column >> row_shift
     const void* tw_col;   // column table of JF_POST_STEP: 4 Shoup pairs per column
+    const void* mz;       // JF_MFMA: matrix block of the radix-16 rounds (ntt_mf16.hpp)
     uint64_t q;
     uint64_t scale_w, scale_wp;  // POST_SCALE constant; with JF_POST_STEP, 2^32 mod q (Shoup pair)
     int64_t s_r, s_c;
@@ -318,6 +319,12 @@ __device__ __forceinline__ void ct_levels_c(ZP z, const M& mod, bool cyc) {
         ct_levels_c<LR, LC, MAXM, L0 + m, M, ZP>(z, mod, false);
     }
 }
+
+}  // namespace nttk
+
+#include "ntt_mf16.hpp"
+
+namespace nttk {
 
 template <class S>
 __device__ __forceinline__ void load4(const S* p, S v[4]);
@@ -653,9 +660,13 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
     uint32_t after = total * (uint32_t)sizeof(V);
     const LdsTw<V> zl{after};
     const TwP<V>* zg = reinterpret_cast<const TwP<V>*>(J.z);
+    const bool mf = std::is_same<M, Mod32>::value && (flags & JF_MFMA);
+    // staged: the CT table (R entries), or the round twiddles of an MFMA pass
+    const TwP<V>* zsrc = mf ? reinterpret_cast<const TwP<V>*>(reinterpret_cast<const char*>(J.mz) + kMfTwOff) : zg;
+    const uint32_t nz = mf ? mf_tw_entries(log_r) : R;
     if (flags & JF_Z_LDS) {
-        for (uint32_t i = tid; i < R; i += T) zl.set(i, zg[i]);
-        after += R * (uint32_t)sizeof(TwP<V>);
+        for (uint32_t i = tid; i < nz; i += T) zl.set(i, zsrc[i]);
+        after += nz * (uint32_t)sizeof(TwP<V>);
     }
     const uint32_t n_lo = 1u << J.lo_bits, n_mid = J.mid_bits ? 1u << J.mid_bits : 0u;
     const uint32_t n_hi = (J.e_mask >> (J.lo_bits + J.mid_bits)) + 1u;
@@ -689,7 +700,15 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
     lds_barrier();
 
     const bool cyc = flags & JF_CT_CYC;
-    if (LR) {
+    if constexpr (std::is_same<M, Mod32>::value) {
+        if (mf) {
+            const bool canon = J.post == POST_NONE;
+            if (flags & JF_Z_LDS) mf_levels(mod, zg, J.mz, zl, log_r, log_c, cyc, canon);
+            else mf_levels(mod, zg, J.mz, zsrc, log_r, log_c, cyc, canon);
+        }
+    }
+    if (mf) {
+    } else if (LR) {
         if (flags & JF_Z_LDS) ct_levels_c<LR, LC, MAXM, 0, M>(zl, mod, cyc);
         else ct_levels_c<LR, LC, MAXM, 0, M>(zg, mod, cyc);
     } else {

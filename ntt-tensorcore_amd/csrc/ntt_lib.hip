@@ -22,7 +22,6 @@
 #include "field_host.hpp"
 #include "ntt_kernels.hpp"
 #include "ntt_launch.hpp"
-#include "ntt_mfma.hpp"
 
 using namespace nttk;
 using namespace ntth;
@@ -104,7 +103,6 @@ __global__ void powers_kernel(uint64_t* out, uint64_t n, const TwP<typename M::V
 // ------------------------------------------------------------------ device limits
 // LDS budget of one workgroup (tile plus, when it fits, the CT table).
 static const uint32_t kLdsMax = kPassLdsMax;
-static const int kThreads = kPassThreads;
 
 // ------------------------------------------------------------------ plans
 enum class Arith { M32, M64 };
@@ -171,9 +169,10 @@ static Split choose_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight) {
 
 struct PassSpec {
     uint32_t log_r = 0, log_c = 0, log_l = 0;  // rows, tile columns, product of earlier radices
-    bool mfma = false;                         // z_off then points at an MFMA table block
+    bool mfma = false;                         // radix-16 rounds on the matrix cores (mz_off)
     bool cyc = false;                          // CT table without the negacyclic factor (Z[2^l] = 1)
     int z_off = -1;
+    int mz_off = -1;                           // matrix block of the radix-16 MFMA rounds (mfma)
     uint32_t post = POST_NONE;
     int lo_off = -1, hi_off = -1, mid_off = -1;
     int row_off = -1;        // row table of the post (see TableBuilder::rows), -1 if none
@@ -313,60 +312,98 @@ static void balanced_digits(int64_t v, int8_t d[4]) {
 
 static int64_t centred(uint64_t v, uint64_t q) { return v > q / 2 ? (int64_t)v - (int64_t)q : (int64_t)v; }
 
-// MFMA table block (layout in ntt_mfma.hpp) for an R-point transform with
-// root w_R, R = 64 or 4096, and negacyclic input factor `fac` = psi_{2R} (1 if cyclic).
-//   R = 64   : stage A matrix F[k][j] = w64^(jk) fac^j.
-//   R = 4096 : stage A F[k1][j1] = w64^(j1 k1) fac^(64 j1); twist w_R^(k1 j2) fac^(j2);
-//              stage B F[k2][j2] = w64^(j2 k2); j = 64 j1 + j2, k = k1 + 64 k2.
-static std::vector<unsigned char> build_mfma_tables(uint32_t log_r, uint64_t wR, uint64_t fac, uint64_t q) {
-    std::vector<unsigned char> blk(kMfmaTableBytes, 0);
-    const uint64_t w64 = powmod(wR, 1ull << (log_r - 6), q);
-    const uint64_t f64 = log_r == 12 ? powmod(fac, 64, q) : fac;
-    int8_t* frag = (int8_t*)blk.data();
-    int32_t* corr = (int32_t*)(blk.data() + kMfmaCorrOff);
-    const int nst = log_r == 12 ? 2 : 1;
-    for (int st = 0; st < nst; st++) {
-        // F[r][j]
-        std::vector<uint64_t> F(64 * 64);
-        for (uint32_t r = 0; r < 64; r++) {
-            const uint64_t wr = powmod(w64, r, q);
-            const uint64_t step = st == 0 ? mulmod(wr, f64, q) : wr;
-            uint64_t t = 1;
-            for (uint32_t j = 0; j < 64; j++) { F[r * 64 + j] = t; t = mulmod(t, step, q); }
-        }
-        for (uint32_t r = 0; r < 64; r++) {
-            uint64_t rs = 0;
-            for (uint32_t j = 0; j < 64; j++) rs = (rs + F[r * 64 + j]) % q;
-            const uint64_t c = mulmod(mulmod(128, 0x01010101ull % q, q), rs, q);
-            int8_t d[4];
-            balanced_digits(centred(c, q), d);
-            for (int i = 0; i < 4; i++) corr[(st * 64 + r) * 4 + i] = d[i];
-        }
-        for (int i = 0; i < 4; i++)
-            for (int mt = 0; mt < 4; mt++)
-                for (int ks = 0; ks < 4; ks++)
-                    for (int lane = 0; lane < 64; lane++)
-                        for (int b = 0; b < 16; b++) {
-                            const uint32_t row = 16 * mt + (lane & 15), j = 16 * ks + 4 * (lane >> 4) + (b >> 2);
-                            const uint64_t v = mulmod(F[row * 64 + j], powmod(2, 8 * (b & 3), q), q);
-                            int8_t d[4];
-                            balanced_digits(centred(v, q), d);
-                            frag[((((size_t)(st * 4 + i) * 4 + mt) * 4 + ks) * 64 + lane) * 16 + b] = d[i];
+// Matrix block of a pass whose radix-16 rounds run on the matrix cores
+// (layout in ntt_mf16.hpp).  From the merged CT table Z of the pass, the
+// round at level l0 maps unit i0's 16 slots by the 4-level butterfly network
+// of ct_unit<4>: x -> U_i0 x.  Every U_i0 factors as F diag(d_i0) with one F
+// for all rounds and units (F[p][v] = w16^(v*br4(p)), the network of block 0
+// without its factors); d_i0[v] = U_i0[0][v] since F[0][v] = 1.  Both are
+// read off the simulated network and the factorisation is checked for every
+// unit, so the MFMA rounds reproduce the VALU rounds' outputs exactly.
+static bool build_mf16_block(uint32_t log_r, const std::vector<uint64_t>& Z, uint64_t q,
+                             std::vector<unsigned char>& blk) {
+    const uint32_t m0 = log_r & 3u;
+    std::vector<uint64_t> F;  // 16 x 16, F[p*16 + v]
+    std::vector<uint64_t> tw;
+    for (uint32_t l0 = m0; l0 + 4 <= log_r; l0 += 4) {
+        for (uint32_t i0 = 0; i0 < (1u << l0); i0++) {
+            uint64_t U[16][16];  // U[p][v]
+            for (uint32_t v = 0; v < 16; v++) {
+                uint64_t x[16] = {};
+                x[v] = 1;
+                for (uint32_t lam = 0; lam < 4; lam++) {
+                    const uint32_t half = 1u << (3 - lam);
+                    for (uint32_t gg = 0; gg < (1u << lam); gg++) {
+                        const uint64_t wt = Z[(1u << (l0 + lam)) + (i0 << lam) + gg];
+                        for (uint32_t v0 = 0; v0 < half; v0++) {
+                            const uint32_t j = gg * 2 * half + v0;
+                            const uint64_t a = x[j], bb = mulmod(x[j + half], wt, q);
+                            x[j] = (a + bb) % q;
+                            x[j + half] = (a + q - bb) % q;
                         }
-    }
-    if (log_r == 12) {
-        uint32_t* tw = (uint32_t*)(blk.data() + kMfmaTwistOff);
-        for (uint32_t k1 = 0; k1 < 64; k1++)
-            for (uint32_t j2 = 0; j2 < 64; j2++) {
-                const uint64_t v = mulmod(powmod(wR, (uint64_t)k1 * j2, q), powmod(fac, j2, q), q);
-                tw[(k1 * 64 + j2) * 2] = (uint32_t)v;
-                tw[(k1 * 64 + j2) * 2 + 1] = (uint32_t)shoup(v, q, 32);
+                    }
+                }
+                for (uint32_t p = 0; p < 16; p++) U[p][v] = x[p];
             }
+            uint64_t dinv[16];
+            for (uint32_t v = 0; v < 16; v++) {
+                if (U[0][v] == 0) return false;
+                tw.push_back(U[0][v]);
+                dinv[v] = invmod(U[0][v], q);
+            }
+            const bool first = F.empty();
+            if (first) F.resize(256);
+            for (uint32_t p = 0; p < 16; p++)
+                for (uint32_t v = 0; v < 16; v++) {
+                    const uint64_t f = mulmod(U[p][v], dinv[v], q);
+                    if (first) F[p * 16 + v] = f;
+                    else if (F[p * 16 + v] != f) return false;
+                }
+        }
     }
-    return blk;
+    if (F.empty()) return false;
+    blk.assign(kMfTwOff + tw.size() * 8, 0);
+    int8_t* frag = (int8_t*)blk.data();
+    int32_t* h = (int32_t*)(blk.data() + kMfFragBytes);
+    uint32_t* hdr = (uint32_t*)(blk.data() + kMfHdrOff);
+    // G[p][(v, b)] = F[p][v] 2^(8b) mod q, K slot 4v + b; lane l holds K = 16 (l >> 4) + i, i < 16
+    auto G = [&](uint32_t p, uint32_t k) -> uint64_t { return mulmod(F[p * 16 + (k >> 2)], 1ull << (8 * (k & 3)), q); };
+    for (uint32_t lane = 0; lane < 64; lane++)
+        for (uint32_t i = 0; i < 16; i++) {
+            int8_t d[4];
+            balanced_digits(centred(G(lane & 15u, 16 * (lane >> 4) + i), q), d);
+            for (int c = 0; c < 4; c++) frag[(c * 64 + lane) * 16 + i] = d[c];
+        }
+    // Row p: y = lo + r + L0 with r = hi * 2^16 (mod q) must represent
+    // sum_K G[p][K] * byte_K; the digits are byte - 128, so plane 2 starts at
+    // T'_p (centred) with L0 + 2^16 T'_p = 128 * rowsum_p (mod q).
+    const uint64_t L0 = (1ull << 28) + (1ull << 21) + (q * 67 + 99) / 100;
+    const uint64_t inv16 = invmod(65536 % q, q);
+    uint32_t qinv = (uint32_t)q;  // q^-1 mod 2^32 by Newton's iteration
+    for (int i = 0; i < 5; i++) qinv *= 2u - (uint32_t)q * qinv;
+    const int64_t cm = centred(((u128)1 << 48) % q, q);
+    hdr[0] = (uint32_t)L0;
+    hdr[1] = (uint32_t)(int32_t)cm;
+    hdr[2] = (uint32_t)(int32_t)cm * qinv;
+    for (uint32_t p = 0; p < 16; p++) {
+        uint64_t rs = 0;
+        for (uint32_t k = 0; k < 64; k++) rs = (rs + G(p, k)) % q;
+        const uint64_t T = mulmod(128, rs, q);
+        const int64_t t = centred(mulmod((T + q - L0 % q) % q, inv16, q), q);
+        for (uint32_t lane = 0; lane < 64; lane++)
+            if (4 * (lane >> 4) <= p && p < 4 * (lane >> 4) + 4) h[lane * 4 + (p & 3)] = (int32_t)t;
+    }
+    uint32_t* twp = (uint32_t*)(blk.data() + kMfTwOff);
+    for (size_t i = 0; i < tw.size(); i++) {
+        twp[2 * i] = (uint32_t)tw[i];
+        twp[2 * i + 1] = (uint32_t)shoup(tw[i], q, 32);
+    }
+    return tw.size() == mf_tw_entries(log_r);
 }
 
-static bool mfma_ok(uint32_t log_r, Arith ar) { return ar == Arith::M32 && (log_r == 6 || log_r == 12); }
+// Passes that may run their radix-16 rounds on the matrix cores: 32-bit
+// arithmetic, R >= 16 and at least 2^8 elements per tile (one 16-unit group).
+static bool mfma_ok(uint32_t log_r, Arith ar) { return ar == Arith::M32 && log_r >= 4; }
 
 // Builds the tables of one transform for a radix split N = R_0 * ... * R_{P-1}.
 // Pass i runs R_i-point transforms; L_i = R_0*...*R_{i-1} and M_i = N / L_i.
@@ -413,7 +450,6 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
     // thread gets whole quads: log_c <= 9 and R >= 4 * 2^(9 - log_c).
     auto post = [&](PassSpec& s, uint64_t base, uint32_t mbits, uint64_t mult, bool first) {
         tb.pow_tables(s, base, mbits, mult);
-        if (s.mfma) return;
         const uint32_t R = 1u << s.log_r;
         if (first && P->npass >= 2 && arith == Arith::M32 && s.log_c <= 9) {
             const uint32_t kstep = 4u << (9 - s.log_c);
@@ -435,15 +471,14 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
         const uint64_t fac = (i == 0 && nega && !inv) ? powmod(p, N >> lr, q) : 1;
         s.cyc = fac == 1;
         // MFMA passes: R = 4096 any width; R = 64 needs >= 16 columns; never single-pass
-        s.mfma = ((sp.mfma >> i) & 1u) && mfma_ok(lr, arith) && sp.npass >= 2 && (lr == 12 || log_n - lr >= 4);
+        const std::vector<uint64_t> Z = ct_table(lr, powmod(w, N >> lr, q), fac, q);
+        s.z_off = tb.add(Z);
+        s.log_c = std::min(s.log_c, max_log_e(arith) > lr ? max_log_e(arith) - lr : 0u);  // tile fits LDS
+        s.mfma = ((sp.mfma >> i) & 1u) && mfma_ok(lr, arith);
         if (s.mfma) {
-            s.z_off = tb.add_raw(build_mfma_tables(lr, powmod(w, N >> lr, q), fac, q));
-            const uint32_t cmax = lr == 12 ? 3u : 8u;  // LDS: 8 columns of 4096, 256 of 64
-            s.log_c = std::min(s.log_c, cmax);
-            if (lr == 6) s.log_c = std::max(s.log_c, std::min(4u, log_n - lr));
-        } else {
-            s.z_off = tb.add(ct_table(lr, powmod(w, N >> lr, q), fac, q));
-            s.log_c = std::min(s.log_c, max_log_e(arith) > lr ? max_log_e(arith) - lr : 0u);  // VALU tile fits LDS
+            std::vector<unsigned char> mb;
+            if (!build_mf16_block(lr, Z, q, mb)) return fail(NTT_ERR_DEVICE, "internal: radix-16 matrix factorisation failed");
+            s.mz_off = tb.add_raw(mb);
         }
         if (i < P->npass - 1) {
             s.post = POST_FOURSTEP;
@@ -497,24 +532,7 @@ static uint32_t post_table_entries(const PassJob& J) {
 
 static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma) {
     const uint32_t R = 1u << J.log_r, C = 1u << J.log_c;
-    uint32_t f = 0;
-    if (mfma) {
-        const int64_t ib = (int64_t)sbytes(sin), ob = (int64_t)sbytes(sout);
-        if (J.s_c == 1) {
-            f |= JF_LOAD_CFAST;
-            if (aligned16(J.in) && (J.in_pstride * ib) % 16 == 0 && C >= 4 && J.s_r % 4 == 0 && J.ncols % 4 == 0)
-                f |= JF_VEC4;
-        }
-        const bool scf = J.o_c == 1 && J.o_lbits >= 2;
-        if (scf) f |= JF_STORE_CFAST;
-        bool vs = aligned16(J.out) && (J.out_pstride * ob) % 16 == 0 && (J.o_hs % 4) == 0;
-        if (scf) vs = vs && C >= 4 && (J.o_r % 4) == 0 && (J.ncols % 4) == 0;
-        else vs = vs && J.o_r == 1 && (J.o_c % 4 == 0 || J.o_lbits == 0);
-        if (vs) f |= JF_VEC4_ST;
-        if (J.post == POST_FOURSTEP) f |= JF_POST4;
-        J.flags = f | JF_MFMA;
-        return;
-    }
+    uint32_t f = mfma ? (uint32_t)JF_MFMA : 0u;
     const bool lcf = J.s_c == 1;
     if (lcf) f |= JF_LOAD_CFAST;
     const bool scf = J.o_c == 1 && J.o_lbits >= 2;
@@ -543,18 +561,15 @@ static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma)
             used += pb;
         }
     }
-    if (used + R * 2 * vbytes(ar) <= kLdsMax) f |= JF_Z_LDS;
+    const uint32_t zn = mfma ? mf_tw_entries(J.log_r) : R;  // staged CT table or MFMA round twiddles
+    if (used + zn * 2 * vbytes(ar) <= kLdsMax) f |= JF_Z_LDS;
     J.flags = f;
 }
 
 static uint32_t job_lds(const PassJob& J, Arith ar) {
     const uint32_t R = 1u << J.log_r;
-    if (J.flags & JF_MFMA) {
-        const uint32_t sc = J.log_r == 12 ? 64u * kMfmaRow + 8u : 72u;
-        return (1u << J.log_c) * sc * 4u;
-    }
     uint32_t b = (R << J.log_c) * vbytes(ar);
-    if (J.flags & JF_Z_LDS) b += R * 2 * vbytes(ar);
+    if (J.flags & JF_Z_LDS) b += ((J.flags & JF_MFMA) ? mf_tw_entries(J.log_r) : R) * 2 * vbytes(ar);
     if (J.flags & JF_POST_ROW) b += R * 2 * vbytes(ar);
     else if (J.flags & JF_POST_LDS) b += post_table_entries(J) * 2 * vbytes(ar);
     return b;
@@ -604,6 +619,7 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         J.tw_mid = s.mid_off >= 0 ? blk + s.mid_off : nullptr;
         J.tw_row = s.row_off >= 0 ? blk + s.row_off : nullptr;
         J.tw_col = s.col_off >= 0 ? blk + s.col_off : nullptr;
+        J.mz = s.mz_off >= 0 ? blk + s.mz_off : nullptr;
         J.row_shift = s.row_shift;
         J.q = P.q;
         J.scale_w = s.scale_w;
@@ -641,8 +657,9 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
             J.in_pstride = J.out_pstride = N;
         }
         set_shifts(J);
-        finish_flags(J, P.arith, src_st, dst_st, s.mfma);
-        if (s.cyc && !s.mfma) J.flags |= JF_CT_CYC;
+        // a radix-16 MFMA round needs one full group of 16 units: >= 2^8 elements per tile
+        finish_flags(J, P.arith, src_st, dst_st, s.mfma && J.log_r + J.log_c >= 8);
+        if (s.cyc) J.flags |= JF_CT_CYC;
         // the stepped store needs staged power tables and row-fast vector stores
         if (J.tw_col && (J.flags & JF_POST_LDS) && !(J.flags & (JF_POST_ROW | JF_STORE_CFAST)) &&
             (J.flags & JF_VEC4_ST))
@@ -653,27 +670,6 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
 }
 
 static uint32_t job_tiles(const PassJob& J) { return J.tiles_per_poly * (J.npoly ? J.npoly : 1); }
-
-template <class Sin, class Sout>
-static ntt_status launch_mfma_t(const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t ntiles, uint32_t lds,
-                                hipStream_t stream) {
-    auto fn = mfma_pass_kernel<Sin, Sout>;
-    static std::once_flag once;
-    std::call_once(once, [&] {
-        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    });
-    // persistent: one resident workgroup per CU (LDS and registers allow one)
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
-    const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cus);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), lds, stream, djobs, njobs, inl, ntiles);
-    HIP_TRY(hipGetLastError());
-    return NTT_OK;
-}
 
 // One launch of the pass kernel for (arith, input storage, output storage).
 template <bool P4>
@@ -699,12 +695,6 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, co
                               const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream,
                               uint32_t shape = 0) {
     if (nblocks == 0) return NTT_OK;
-    if (kind & JF_MFMA) {
-        if (sin == Store::U32 && sout == Store::U32) return launch_mfma_t<uint32_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
-        if (sin == Store::U64 && sout == Store::U32) return launch_mfma_t<uint64_t, uint32_t>(djobs, njobs, inl, nblocks, lds, stream);
-        if (sin == Store::U32 && sout == Store::U64) return launch_mfma_t<uint32_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
-        return launch_mfma_t<uint64_t, uint64_t>(djobs, njobs, inl, nblocks, lds, stream);
-    }
     if (shape) {
         const bool p4 = kind & JF_POST4;
         hipError_t e = hipErrorInvalidValue;
@@ -730,7 +720,7 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, co
 // in LDS, and a shape that has an instance.
 static uint32_t pass_shape_of(const PassJob& J, Arith ar, Store sin, Store sout) {
     if (ar != Arith::M32 || sin != Store::U32 || sout != Store::U32) return 0;
-    if ((J.flags & JF_MFMA) || !(J.flags & JF_VEC4) || !(J.flags & JF_VEC4_ST)) return 0;
+    if (!(J.flags & JF_VEC4) || !(J.flags & JF_VEC4_ST)) return 0;
     if (J.post == POST_FOURSTEP && !(J.flags & JF_POST_LDS)) return 0;
     if (J.ncols % (1u << J.log_c) != 0) return 0;
     const uint32_t key = (J.log_r << 8) | J.log_c;

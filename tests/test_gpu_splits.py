@@ -1,5 +1,6 @@
 """Every plan shape (1..4 passes, uneven radices, narrow and wide tiles,
-uint32 and uint64 storage, 31- and 62-bit primes) against the oracle."""
+uint32 and uint64 storage, 31- and 62-bit primes) against the oracle, with
+VALU rounds only and with the radix-16 rounds on the matrix cores."""
 import numpy as np
 import pytest
 
@@ -23,15 +24,21 @@ def ctx(nt):
     c.destroy()
 
 
+def _mask(split):
+    """Matrix-core rounds in every pass with R >= 16."""
+    return sum(1 << i for i, r in enumerate(split) if r >= 4)
+
+
 @pytest.mark.parametrize("log_n,split,log_e", SPLITS)
 @pytest.mark.parametrize("kind", ["cyc", "icyc", "nega", "inega"])
-def test_split_u32(nt, orc, ctx, log_n, split, log_e, kind):
+@pytest.mark.parametrize("mf", [False, True])
+def test_split_u32(nt, orc, ctx, log_n, split, log_e, kind, mf):
     n = 1 << log_n
     x = orc.random_poly(n, BB, 7 * log_n + len(split))
     flags = {"cyc": 0, "icyc": nt.FLAG_INVERSE, "nega": nt.FLAG_NEGACYCLIC,
              "inega": nt.FLAG_NEGACYCLIC | nt.FLAG_INVERSE}[kind]
     buf = nt.DeviceBuffer.from_array(x.astype(np.uint32))
-    ctx.transform_split(buf.ptr, BB, log_n, flags, split, log_e)
+    ctx.transform_split(buf.ptr, BB, log_n, flags, split, log_e, mfma=_mask(split) if mf else 0)
     nt.device_sync()
     got = buf.download(np.uint32, n).astype(np.uint64)
     assert np.array_equal(got, orc.transform(x, BB, "nega" in kind, kind.startswith("i")))
@@ -40,12 +47,16 @@ def test_split_u32(nt, orc, ctx, log_n, split, log_e, kind):
 @pytest.mark.parametrize("log_n,split,log_e", [(8, [4, 4], 8), (12, [6, 6], 10), (13, [5, 4, 4], 13),
                                                (16, [8, 8], 12), (18, [6, 6, 6], 12)])
 @pytest.mark.parametrize("q", [BB, P62])
-def test_split_u64(nt, orc, ctx, log_n, split, log_e, q):
+@pytest.mark.parametrize("mf", [False, True])
+def test_split_u64(nt, orc, ctx, log_n, split, log_e, q, mf):
+    if mf and q == P62:
+        pytest.skip("62-bit arithmetic has no matrix-core rounds")
     n = 1 << log_n
     x = orc.random_poly(n, q, 3 * log_n)
     for flags, nega, inv in [(0, False, False), (3, True, True), (2, True, False)]:
         buf = nt.DeviceBuffer.from_array(x)
-        ctx.transform_split(buf.ptr, q, log_n, flags | nt.FLAG_U64, split, min(log_e, 13 if q == P62 else 14))
+        ctx.transform_split(buf.ptr, q, log_n, flags | nt.FLAG_U64, split, min(log_e, 13 if q == P62 else 14),
+                            mfma=_mask(split) if mf else 0)
         nt.device_sync()
         assert np.array_equal(buf.download(np.uint64, n), orc.transform(x, q, nega, inv)), (flags, split)
         buf.free()
@@ -61,7 +72,9 @@ def test_split_rejects_bad_shapes(nt, ctx):
 
 MFMA_SPLITS = [
     (18, [12, 6], 14, 1), (18, [12, 6], 14, 3), (18, [6, 12], 14, 2), (16, [6, 10], 12, 1),
-    (24, [12, 12], 14, 3), (20, [12, 8], 14, 1), (12, [6, 6], 10, 3), (22, [6, 6, 10], 12, 3),
+    (24, [12, 12], 14, 3), (24, [12, 12], 13, 3), (24, [8, 8, 8], 13, 7), (24, [8, 8, 8], 12, 5),
+    (20, [12, 8], 14, 1), (20, [10, 10], 13, 3), (12, [6, 6], 10, 3), (22, [11, 11], 14, 3),
+    (22, [6, 6, 10], 12, 3),
 ]
 
 
@@ -69,8 +82,9 @@ MFMA_SPLITS = [
 @pytest.mark.parametrize("kind", ["cyc", "icyc", "nega", "inega"])
 @pytest.mark.parametrize("u64", [False, True])
 def test_mfma_split(nt, orc, ctx, log_n, split, log_e, mask, kind, u64):
-    """INT8-MFMA passes (R = 64 and 4096) against the oracle, bit-exact; at
-    n = 2^24 through size-independent properties (round trip, sampled outputs)."""
+    """Radix-16 rounds on the matrix cores in the passes of `mask`, against the
+    oracle, bit-exact; at n >= 2^22 through size-independent properties (round
+    trip through the VALU path, outputs sampled from the defining sums)."""
     if log_n >= 22 and (u64 or kind in ("icyc", "inega")):
         pytest.skip("one direction per kind at the largest sizes")
     q = BB
@@ -112,3 +126,19 @@ def _pw(r, n, q):
         rm = rm * rm % q
         m *= 2
     return p
+
+
+@pytest.mark.parametrize("q,log_n,split,log_e", [(1056440321, 16, [8, 8], 13), (7340033, 15, [7, 8], 12),
+                                                 (65537, 14, [6, 8], 14), (2147352577, 16, [12, 4], 13)])
+@pytest.mark.parametrize("kind", ["cyc", "inega"])
+def test_mfma_primes(nt, orc, ctx, q, log_n, split, log_e, kind):
+    """Matrix-core rounds for primes of every size class: below 2^30 the last
+    round reduces its outputs by a Shoup step instead of one subtraction."""
+    n = 1 << log_n
+    x = orc.random_poly(n, q, 5 * log_n)
+    flags = 0 if kind == "cyc" else nt.FLAG_NEGACYCLIC | nt.FLAG_INVERSE
+    buf = nt.DeviceBuffer.from_array(x.astype(np.uint32))
+    ctx.transform_split(buf.ptr, q, log_n, flags, split, log_e, mfma=_mask(split))
+    nt.device_sync()
+    got = buf.download(np.uint32, n).astype(np.uint64)
+    assert np.array_equal(got, orc.transform(x, q, kind == "inega", kind == "inega"))
