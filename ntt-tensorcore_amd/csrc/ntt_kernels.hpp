@@ -38,10 +38,9 @@ struct Mod32 {
     using V = uint32_t;
     uint32_t q;
     __device__ __forceinline__ V reduce2(V r) const { return min(r, r - q); }  // [0,2q) -> [0,q)
-    __device__ __forceinline__ V mul(V x, TwP<V> t) const {
-        V hi = __umulhi(x, t.wp);
-        return reduce2(x * t.w - hi * q);
-    }
+    __device__ __forceinline__ V mul(V x, TwP<V> t) const { return reduce2(mul_lazy(x, t)); }
+    // x * w mod q in [0, 2q), for any x < 2^32
+    __device__ __forceinline__ V mul_lazy(V x, TwP<V> t) const { return x * t.w - __umulhi(x, t.wp) * q; }
     __device__ __forceinline__ V add(V a, V b) const { return reduce2(a + b); }
     // a - b wraps above 2^32 - q exactly when a < b, and then t + q = a - b + q
     // is the smaller of the two (2q < 2^32)
@@ -74,10 +73,8 @@ struct Mod64 {
     using V = uint64_t;
     uint64_t q;
     __device__ __forceinline__ V reduce2(V r) const { return r >= q ? r - q : r; }
-    __device__ __forceinline__ V mul(V x, TwP<V> t) const {
-        V hi = __umul64hi(x, t.wp);
-        return reduce2(x * t.w - hi * q);
-    }
+    __device__ __forceinline__ V mul(V x, TwP<V> t) const { return reduce2(mul_lazy(x, t)); }
+    __device__ __forceinline__ V mul_lazy(V x, TwP<V> t) const { return x * t.w - __umul64hi(x, t.wp) * q; }
     __device__ __forceinline__ V add(V a, V b) const { return reduce2(a + b); }
     __device__ __forceinline__ V sub(V a, V b) const { return reduce2(a - b + q); }
 };
@@ -108,6 +105,8 @@ enum JobFlags : uint32_t {
     JF_CT_CYC = 256u,     // CT table is cyclic (Z[2^l] = 1): round 0 skips products by 1
     JF_POST_ROW = 512u,   // post from the row table (one factor per row k), staged in LDS
     JF_POST_STEP = 1024u, // row-fast first-pass store with per-column stepped factors (tw_col)
+    JF_LAZY_OUT = 2048u,  // row-table post may leave values in [0, 2q): the next pass's matrix-core
+                          // round takes any 32-bit representative
 };
 
 struct PassJob {
@@ -364,8 +363,10 @@ constexpr int kUnroll = 8;
 //   PRED per-element bounds predicates (partial tiles, tiles smaller than a sweep),
 //   SWT  tables are staged in LDS at tsw(i),
 //   LR, LC the tile shape log2 R, log2 C when fixed at compile time (LR = 0:
-//   taken from the job).
-template <int PER, int POST, int NTAB, int QM, bool PRED, bool SWT, int LR, int LC, class M, class Sout, class TP>
+//   taken from the job),
+//   LZ   (QM 3) outputs in [0, 2q) instead of canonical (JF_LAZY_OUT).
+template <int PER, int POST, int NTAB, int QM, bool PRED, bool SWT, int LR, int LC, class M, class Sout,
+          bool LZ = false, class TP>
 __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint32_t poly, uint32_t col0, TP tl, TP tm,
                                            TP th) {
     using V = typename M::V;
@@ -432,10 +433,11 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint3
                     if (cf) {
                         const TwP<V> f = tl[k];
 #pragma unroll
-                        for (int e = 0; e < PER; e++) x[e] = mod.mul(x[e], f);
+                        for (int e = 0; e < PER; e++) x[e] = LZ ? mod.mul_lazy(x[e], f) : mod.mul(x[e], f);
                     } else {
 #pragma unroll
-                        for (int e = 0; e < PER; e++) x[e] = mod.mul(x[e], tl[k + (uint32_t)e]);
+                        for (int e = 0; e < PER; e++)
+                            x[e] = LZ ? mod.mul_lazy(x[e], tl[k + (uint32_t)e]) : mod.mul(x[e], tl[k + (uint32_t)e]);
                     }
                 } else if (QM == 1) {
                     const uint32_t ex = expo(k, cg);
@@ -526,9 +528,9 @@ __device__ __forceinline__ void store_loop_step(const PassJob& J, const Mod32& m
         V x[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) x[j] = lds_ld<V>(su ^ jx[j]);
-        x[1] = mod.mul(x[1], B1);
-        x[2] = mod.mul(x[2], B2);
-        x[3] = mod.mul(x[3], B3);
+        x[1] = mod.mul_lazy(x[1], B1);  // mont32 takes any representative below 2^32
+        x[2] = mod.mul_lazy(x[2], B2);
+        x[3] = mod.mul_lazy(x[3], B3);
 #pragma unroll
         for (int j = 0; j < 4; j++) x[j] = mont32(x[j], G, q, qinv);
         G = mod.mul(G, Wb);
@@ -543,6 +545,7 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
     const uint32_t post = J.post;
     if (P4 && post == POST_FOURSTEP && SWT && (J.flags & JF_POST_ROW)) {
         if (pred) store_loop<PER, 2, 2, 3, true, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
+        else if (J.flags & JF_LAZY_OUT) store_loop<PER, 2, 2, 3, false, SWT, LR, LC, M, Sout, true>(J, mod, poly, col0, tl, tm, th);
         else store_loop<PER, 2, 2, 3, false, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
     } else if (std::is_same<M, Mod32>::value && PER == 4 && P4 && post == POST_FOURSTEP && SWT && !pred &&
                (J.flags & JF_POST_STEP)) {
@@ -640,54 +643,126 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
     }
 }
 
+// LDS behind the tile: the CT table (R entries) or an MFMA pass's round
+// twiddles, then the post-multiply tables (lo, mid, hi) or the tile's row
+// table.  Offsets depend on the job only.
+template <class V>
+struct Staged {
+    LdsTw<V> zl, pl, pm, ph;
+};
+
+template <class M, bool P4, int LR>
+__device__ __forceinline__ Staged<typename M::V> staged_of(const PassJob& J) {
+    using V = typename M::V;
+    const uint32_t log_r = LR ? (uint32_t)LR : J.log_r;
+    const bool mf = std::is_same<M, Mod32>::value && (J.flags & JF_MFMA);
+    uint32_t after = (1u << (log_r + J.log_c)) * (uint32_t)sizeof(V);
+    Staged<V> t;
+    t.zl = LdsTw<V>{after};
+    if (J.flags & JF_Z_LDS) after += (mf ? mf_tw_entries(log_r) : (1u << log_r)) * (uint32_t)sizeof(TwP<V>);
+    t.pl = LdsTw<V>{after};
+    t.pm = t.pl + (1u << J.lo_bits);
+    t.ph = t.pm + (J.mid_bits ? 1u << J.mid_bits : 0u);
+    return t;
+}
+
+// Copies the job's tables into LDS: everything (job tables and the row
+// table of column col0), or (row_only) just the row table.
+template <class M, bool P4, int LR>
+__device__ __forceinline__ void stage_tables(const PassJob& J, const Staged<typename M::V>& t, uint32_t col0,
+                                             bool row_only) {
+    using V = typename M::V;
+    const uint32_t log_r = LR ? (uint32_t)LR : J.log_r;
+    const uint32_t R = 1u << log_r;
+    const uint32_t T = kNT, tid = threadIdx.x;
+    const uint32_t flags = J.flags;
+    if (!row_only && (flags & JF_Z_LDS)) {
+        const bool mf = std::is_same<M, Mod32>::value && (flags & JF_MFMA);
+        const TwP<V>* zsrc = mf ? reinterpret_cast<const TwP<V>*>(reinterpret_cast<const char*>(J.mz) + kMfTwOff)
+                                : reinterpret_cast<const TwP<V>*>(J.z);
+        const uint32_t nz = mf ? mf_tw_entries(log_r) : R;
+        for (uint32_t i = tid; i < nz; i += T) t.zl.set(i, zsrc[i]);
+    }
+    if (P4 && (flags & JF_POST_ROW)) {
+        const TwP<V>* gr = reinterpret_cast<const TwP<V>*>(J.tw_row) + ((size_t)(col0 >> J.row_shift) << log_r);
+        for (uint32_t i = tid; i < R; i += T) t.pl.set(i, gr[i]);
+    } else if (P4 && !row_only && (flags & JF_POST_LDS)) {
+        const uint32_t n_lo = 1u << J.lo_bits, n_mid = J.mid_bits ? 1u << J.mid_bits : 0u;
+        const uint32_t n_hi = (J.e_mask >> (J.lo_bits + J.mid_bits)) + 1u;
+        const TwP<V>* gl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
+        const TwP<V>* gm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
+        const TwP<V>* gh = reinterpret_cast<const TwP<V>*>(J.tw_hi);
+        for (uint32_t i = tid; i < n_lo; i += T) t.pl.set(tsw(i), gl[i]);
+        for (uint32_t i = tid; i < n_mid; i += T) t.pm.set(tsw(i), gm[i]);
+        for (uint32_t i = tid; i < n_hi; i += T) t.ph.set(tsw(i), gh[i]);
+    }
+}
+
+// The Cooley-Tukey levels of the tile in LDS (VALU rounds, or radix-16
+// rounds on the matrix cores), then the store with the pass's post-multiply.
+template <class M, class Sout, int MAXM, bool P4, int LR, int LC>
+__device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<typename M::V>& t, uint32_t poly,
+                                             uint32_t col0, bool full) {
+    using V = typename M::V;
+    const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
+    const uint32_t total = 1u << (log_r + log_c);
+    const uint32_t T = kNT;
+    const M mod{(decltype(M::q))J.q};
+    const uint32_t flags = J.flags;
+    const TwP<V>* zg = reinterpret_cast<const TwP<V>*>(J.z);
+    const bool mf = std::is_same<M, Mod32>::value && (flags & JF_MFMA);
+    const bool cyc = flags & JF_CT_CYC;
+    if constexpr (std::is_same<M, Mod32>::value) {
+        if (mf) {
+            const TwP<V>* twg = reinterpret_cast<const TwP<V>*>(reinterpret_cast<const char*>(J.mz) + kMfTwOff);
+            const bool canon = J.post == POST_NONE;
+            MfMat mat;
+            mf_load_mat(J.mz, mat);
+            if (flags & JF_Z_LDS) mf_levels(mod, zg, mat, t.zl, log_r, log_c, cyc, canon);
+            else mf_levels(mod, zg, mat, twg, log_r, log_c, cyc, canon);
+        }
+    }
+    if (mf) {
+    } else if (LR) {
+        if (flags & JF_Z_LDS) ct_levels_c<LR, LC, MAXM, 0, M>(t.zl, mod, cyc);
+        else ct_levels_c<LR, LC, MAXM, 0, M>(zg, mod, cyc);
+    } else {
+        if (flags & JF_Z_LDS) ct_levels<MAXM, M>(t.zl, mod, log_r, log_c, cyc);
+        else ct_levels<MAXM, M>(zg, mod, log_r, log_c, cyc);
+    }
+
+    const bool pred_s = !full || total < ((flags & JF_VEC4_ST) ? 4u : 1u) * T;
+    const TwP<V>* tl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
+    const TwP<V>* tm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
+    const TwP<V>* th = reinterpret_cast<const TwP<V>*>(J.tw_hi);
+    if (LR) {  // shape-specialised: vector stores, full tiles, LDS post tables
+        store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, false, t.pl, t.pm, t.ph);
+    } else if (P4 && (flags & JF_POST_LDS)) {
+        if (flags & JF_VEC4_ST) store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, t.pl, t.pm, t.ph);
+        else store_dispatch<1, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, t.pl, t.pm, t.ph);
+    } else {
+        if (flags & JF_VEC4_ST) store_dispatch<4, P4, false, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, tl, tm, th);
+        else store_dispatch<1, P4, false, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, tl, tm, th);
+    }
+}
+
 template <class M, class Sin, class Sout, int MAXM, bool P4, int LR, int LC>
 __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job) {
     using V = typename M::V;
     const uint32_t poly = tile_in_job / J.tiles_per_poly;
     const uint32_t tile = tile_in_job - poly * J.tiles_per_poly;
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
-    const uint32_t R = 1u << log_r, C = 1u << log_c;
+    const uint32_t C = 1u << log_c;
     const uint32_t col0 = tile << log_c;
-    const uint32_t total = R << log_c;
-    const uint32_t ncols = J.ncols;
-    const bool full = col0 + C <= ncols;  // every column of the tile exists
-    const uint32_t T = kNT, tid = threadIdx.x;
-    const M mod{(decltype(M::q))J.q};
+    const uint32_t total = 1u << (log_r + log_c);
+    const bool full = col0 + C <= J.ncols;  // every column of the tile exists
     const uint32_t flags = J.flags;
-
-    // LDS: the tile at offset 0, then the tables staged behind it: the CT
-    // table (R entries), then the post-multiply tables (lo, mid, hi).
-    uint32_t after = total * (uint32_t)sizeof(V);
-    const LdsTw<V> zl{after};
-    const TwP<V>* zg = reinterpret_cast<const TwP<V>*>(J.z);
-    const bool mf = std::is_same<M, Mod32>::value && (flags & JF_MFMA);
-    // staged: the CT table (R entries), or the round twiddles of an MFMA pass
-    const TwP<V>* zsrc = mf ? reinterpret_cast<const TwP<V>*>(reinterpret_cast<const char*>(J.mz) + kMfTwOff) : zg;
-    const uint32_t nz = mf ? mf_tw_entries(log_r) : R;
-    if (flags & JF_Z_LDS) {
-        for (uint32_t i = tid; i < nz; i += T) zl.set(i, zsrc[i]);
-        after += nz * (uint32_t)sizeof(TwP<V>);
-    }
-    const uint32_t n_lo = 1u << J.lo_bits, n_mid = J.mid_bits ? 1u << J.mid_bits : 0u;
-    const uint32_t n_hi = (J.e_mask >> (J.lo_bits + J.mid_bits)) + 1u;
-    const LdsTw<V> pl{after};
-    const LdsTw<V> pm = pl + n_lo;
-    const LdsTw<V> ph = pm + n_mid;
-    if (P4 && (flags & JF_POST_ROW)) {
-        const TwP<V>* gr = reinterpret_cast<const TwP<V>*>(J.tw_row) + ((size_t)(col0 >> J.row_shift) << log_r);
-        for (uint32_t i = tid; i < R; i += T) pl.set(i, gr[i]);
-    } else if (P4 && (flags & JF_POST_LDS)) {
-        const TwP<V>* gl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
-        const TwP<V>* gm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
-        const TwP<V>* gh = reinterpret_cast<const TwP<V>*>(J.tw_hi);
-        for (uint32_t i = tid; i < n_lo; i += T) pl.set(tsw(i), gl[i]);
-        for (uint32_t i = tid; i < n_mid; i += T) pm.set(tsw(i), gm[i]);
-        for (uint32_t i = tid; i < n_hi; i += T) ph.set(tsw(i), gh[i]);
-    }
+    const Staged<V> t = staged_of<M, P4, LR>(J);
+    stage_tables<M, P4, LR>(J, t, col0, false);
 
     // ---- load ----
     // Predicates are needed for partial tiles and for tiles smaller than one sweep.
-    const bool pred_l = !full || total < ((flags & JF_VEC4) ? 4u : 1u) * T;
+    const bool pred_l = !full || total < ((flags & JF_VEC4) ? 4u : 1u) * kNT;
     if (LR) {  // shape-specialised launches are full tiles with vector loads
         load_loop<4, false, LR, LC, Sin, V>(J, poly, col0);
     } else if (flags & JF_VEC4) {
@@ -698,37 +773,7 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
         else load_loop<1, false, LR, LC, Sin, V>(J, poly, col0);
     }
     lds_barrier();
-
-    const bool cyc = flags & JF_CT_CYC;
-    if constexpr (std::is_same<M, Mod32>::value) {
-        if (mf) {
-            const bool canon = J.post == POST_NONE;
-            if (flags & JF_Z_LDS) mf_levels(mod, zg, J.mz, zl, log_r, log_c, cyc, canon);
-            else mf_levels(mod, zg, J.mz, zsrc, log_r, log_c, cyc, canon);
-        }
-    }
-    if (mf) {
-    } else if (LR) {
-        if (flags & JF_Z_LDS) ct_levels_c<LR, LC, MAXM, 0, M>(zl, mod, cyc);
-        else ct_levels_c<LR, LC, MAXM, 0, M>(zg, mod, cyc);
-    } else {
-        if (flags & JF_Z_LDS) ct_levels<MAXM, M>(zl, mod, log_r, log_c, cyc);
-        else ct_levels<MAXM, M>(zg, mod, log_r, log_c, cyc);
-    }
-
-    const bool pred_s = !full || total < ((flags & JF_VEC4_ST) ? 4u : 1u) * T;
-    const TwP<V>* tl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
-    const TwP<V>* tm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
-    const TwP<V>* th = reinterpret_cast<const TwP<V>*>(J.tw_hi);
-    if (LR) {  // shape-specialised: vector stores, full tiles, LDS post tables
-        store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, false, pl, pm, ph);
-    } else if (P4 && (flags & JF_POST_LDS)) {
-        if (flags & JF_VEC4_ST) store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, pl, pm, ph);
-        else store_dispatch<1, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, pl, pm, ph);
-    } else {
-        if (flags & JF_VEC4_ST) store_dispatch<4, P4, false, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, tl, tm, th);
-        else store_dispatch<1, P4, false, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, tl, tm, th);
-    }
+    ct_and_store<M, Sout, MAXM, P4, LR, LC>(J, t, poly, col0, full);
 }
 
 // Workgroup index -> logical tile so that consecutive tiles run on the same
@@ -769,6 +814,13 @@ __global__ void __launch_bounds__(NT) pass_kernel(const PassJob* __restrict__ jo
     run_pass<M, Sin, Sout, MAXM, P4, LR, LC>(J, tile - J.tile_begin);
 }
 
+// Persistent form of the shape-specialised pass kernel (32-bit coefficients
+// in and out, full tiles, vector loads and stores): workgroup b walks the
+// tiles xcd_remap(b + k*grid) (grid a multiple of 8, so every tile of a
+// workgroup lies in its XCD's contiguous range) and, once its current tile
+// is in LDS, issues the global loads of its next tile into registers, which
+// arrive while the current tile runs its rounds and its store.  Job tables
+// are staged when the job changes, a row table when the tile's row changes.
 // ---- small elementwise kernels ----
 
 // r = a*b mod q with q < 2^31 on uint64 storage (poly_mul_kernel semantics,

@@ -80,8 +80,9 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
                                          uint32_t log_c) {
     const uint32_t sh = log_r - l0 - 4u + log_c;  // log2 of the word stride of v within a unit
     const uint32_t ngroups = 1u << (log_r + log_c - 8u);
-    const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, n = lane & 15u;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u, g = lane >> 4, n = lane & 15u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t lowmask = (1u << sh) - 1u;
     // row-major word of unit u's element 0: u's low sh bits stay, the rest moves up by 4
     auto word = [&](uint32_t u) -> uint32_t { return (u & lowmask) | ((u & ~lowmask) << 4); };
@@ -131,11 +132,11 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
 
 // All log_r levels of the tile: a VALU round of log_r mod 4 levels first
 // (merged CT table zg, as ct_levels), then radix-16 rounds on the matrix
-// cores.  mb: the pass's matrix block; twl: its round twiddle tables (LDS or
+// cores.  M: the pass's matrix (mf_load_mat); twl: its round twiddle tables (LDS or
 // global); cyc: cyclic table, so round 0's twiddles are all 1; canon_out:
 // the tile is stored without a post-multiply, so the outputs must be canonical.
 template <class ZP, class TP>
-__device__ __forceinline__ void mf_levels(const Mod32& mod, ZP zg, const void* mb, TP twl, uint32_t log_r,
+__device__ __forceinline__ void mf_levels(const Mod32& mod, ZP zg, const MfMat& M, TP twl, uint32_t log_r,
                                           uint32_t log_c, bool cyc, bool canon_out) {
     const uint32_t m0 = log_r & 3u;
     if (m0) {
@@ -146,8 +147,6 @@ __device__ __forceinline__ void mf_levels(const Mod32& mod, ZP zg, const void* m
         }
         lds_barrier();
     }
-    MfMat M;
-    mf_load_mat(mb, M);
     const uint32_t q = mod.q;
     // canonical output only where no post-multiply follows (which takes any
     // representative below 2^32)
