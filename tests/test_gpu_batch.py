@@ -11,6 +11,24 @@ pytestmark = pytest.mark.gpu
 BB = 2013265921
 
 
+def _mfma_plans(nt, tmp_path, log_ns, elems):
+    """Installs a plan table that runs every size of a batch of `elems`
+    elements with radix-16 rounds on the matrix cores (the table keys plans
+    by log2 n and log2 of the batch's elements): one pass up to 2^14, else
+    [8, log n - 8]."""
+    key = int(elems).bit_length() - 1
+    lines = []
+    for ln in sorted(set(log_ns)):
+        if ln <= 14:
+            lines.append(f"0 {ln} {key} 1 {ln} 0 0 0 {max(ln, 13)} 1")
+        else:
+            lines.append(f"0 {ln} {key} 2 8 {ln - 8} 0 0 13 3")
+    f = tmp_path / "mfma_plans.txt"
+    f.write_text("\n".join(lines) + "\n")
+    nt.autotune_clear()
+    nt.autotune_load(str(f))
+
+
 def _upload_polys(nt, polys, u64):
     dt = np.uint64 if u64 else np.uint32
     sizes = [p.size * (8 if u64 else 4) for p in polys]
@@ -27,8 +45,11 @@ def _download_polys(nt, buf, offs, sizes, u64):
 
 
 @pytest.mark.parametrize("u64", [False, True])
-def test_uneven_batch_matches_oracle_and_graph_replays(nt, orc, u64):
+@pytest.mark.parametrize("mf", [False, True])
+def test_uneven_batch_matches_oracle_and_graph_replays(nt, orc, u64, mf, tmp_path):
     jobs = uneven_batch(count=48, seed=4, log_min=1, log_max=16)
+    if mf:
+        _mfma_plans(nt, tmp_path, [j["log_n"] for j in jobs], sum(1 << j["log_n"] for j in jobs))
     polys = [orc.random_poly(1 << j["log_n"], j["q"], j["seed"]) for j in jobs]
     expect = [orc.transform(p, j["q"], j["nega"], j["inverse"]) for p, j in zip(polys, jobs)]
     buf, offs = _upload_polys(nt, polys, u64)
@@ -57,10 +78,23 @@ def test_uneven_batch_matches_oracle_and_graph_replays(nt, orc, u64):
             assert np.array_equal(g, e), (rep, i)
     batch.destroy()
     ctx.destroy()
+    nt.autotune_clear()
 
 
-def test_alone_equals_in_mix(nt, orc):
+@pytest.mark.parametrize("mf", [False, True])
+def test_alone_equals_in_mix(nt, orc, mf, tmp_path):
     jobs = uneven_batch(count=24, seed=11, log_min=6, log_max=17)
+    if mf:
+        # the same splits for the batch and for each polynomial alone (a
+        # single transform keys its plan by its own size)
+        log_ns = [j["log_n"] for j in jobs]
+        _mfma_plans(nt, tmp_path, log_ns, sum(1 << ln for ln in log_ns))
+        key = sum(1 << ln for ln in log_ns).bit_length() - 1
+        extra = tmp_path / "alone.txt"
+        extra.write_text("".join(f"0 {ln} {ln} " + ("1 %d 0 0 0 %d 1\n" % (ln, max(ln, 13)) if ln <= 14 else
+                                                    "2 8 %d 0 0 13 3\n" % (ln - 8)) for ln in sorted(set(log_ns))
+                                 if ln != key))
+        nt.autotune_load(str(extra))
     polys = [orc.random_poly(1 << j["log_n"], j["q"], j["seed"]) for j in jobs]
     ctx = nt.Context()
     flags = [(nt.FLAG_INVERSE if j["inverse"] else 0) | (nt.FLAG_NEGACYCLIC if j["nega"] else 0) for j in jobs]
@@ -75,16 +109,21 @@ def test_alone_equals_in_mix(nt, orc):
         nt.device_sync()
         alone = one.download(np.uint32, p.size).astype(np.uint64)
         assert np.array_equal(alone, mixed[i]), i
+        assert np.array_equal(alone, orc.transform(p, j["q"], j["nega"], j["inverse"])), i
         one.free()
     batch.destroy()
     ctx.destroy()
+    nt.autotune_clear()
 
 
-def test_rns_limbs_one_batch(nt, orc):
-    """cfg3 shape at reduced n: 24 limbs (the 24 largest primes < 2^31 with
-    q = 1 mod 2^17), negacyclic, one prepared batch."""
-    log_n = 12
+@pytest.mark.parametrize("log_n,mf", [(12, False), (12, True), (16, True)])
+def test_rns_limbs_one_batch(nt, orc, log_n, mf, tmp_path):
+    """cfg3 shape (24 limbs: the 24 largest primes < 2^31 with q = 1 mod
+    2^17), negacyclic, one prepared batch: 24 different matrix blocks in one
+    launch on the matrix-core path."""
     primes = rns_primes()
+    if mf:
+        _mfma_plans(nt, tmp_path, [log_n], len(primes) << log_n)
     polys = [orc.random_poly(1 << log_n, q, 0x5EED ^ i) for i, q in enumerate(primes)]
     buf, offs = _upload_polys(nt, polys, False)
     ctx = nt.Context()
@@ -96,3 +135,4 @@ def test_rns_limbs_one_batch(nt, orc):
         assert np.array_equal(g, orc.ntt_negacyclic(p, q)), i
     batch.destroy()
     ctx.destroy()
+    nt.autotune_clear()
