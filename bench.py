@@ -144,6 +144,31 @@ def extra_latency_4096(ctx, stream):
     return {"p50_us_host_roundtrip": float(np.percentile(lat, 50)), "device_us_per_pair": ms * 1e3 / 200}
 
 
+# The reference's only published measurement of this path: ntt_forward_optimized,
+# forward cyclic BabyBear, one uint64_t polynomial per call, median us on an RTX
+# 5060 (reference README.md:120-126).  Context only: other hardware.
+REFERENCE_FWD_US = {12: 15.01, 14: 27.17, 16: 62.43, 17: 104.29, 18: 257.47, 19: 491.14, 20: 963.65}
+
+
+def extra_reference_table(stream):
+    """The reference's published workload on this library's drop-in uint64_t
+    ABI (ntt_c_forward, variant optimized): one forward cyclic transform per
+    call; device time per call from HIP events over 50 calls, median of 5."""
+    out = {}
+    for log_n, ref_us in REFERENCE_FWD_US.items():
+        n = 1 << log_n
+        cfg = nt.Config(n, BB)
+        buf = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 42))
+        step = lambda: cfg.forward(buf, stream=stream)  # noqa: E731
+        reps = [timed(step, 50, 5, stream) * 1e3 / 50 for _ in range(5)]
+        cfg.cleanup()
+        buf.free()
+        us = float(np.median(reps))
+        out[f"2^{log_n}"] = {"us": us, "Gbfly_s": W.butterflies(log_n) / us / 1e3,
+                             "reference_rtx5060_us": ref_us}
+    return out
+
+
 def extra_2_20(ctx, stream):
     out = {}
     for batch in (1, 16):
@@ -299,7 +324,7 @@ def tune_all(ctx, args):
     def rec(name, q, log_ns, elems):
         for r in nt.autotune(ctx, q, log_ns, elems, reps=4):
             out[f"{name}:2^{r['log_n']}"] = {k: r[k] for k in ("split", "log_e", "mfma_mask", "us", "heuristic_us")}
-    rec("single", BB, [args.log_n, 20], 1)
+    rec("single", BB, sorted({args.log_n, 12, 13, 14, 15, 16, 17, 18, 19, 20}), 1)
     rec("batch_2^24", BB, [20, 16], 1 << 24)
     rec("rns", BB, [16], 24 << 16)
     rec("uneven", BB, list(range(10, 19)), 1 << 24)
@@ -347,6 +372,7 @@ def main():
     extras = {}
     if not args.no_extras:
         for name, fn in [("cyclic_4096", lambda: extra_latency_4096(ctx, stream)),
+                         ("reference_table", lambda: extra_reference_table(stream)),
                          ("negacyclic_2_20", lambda: extra_2_20(ctx, stream)),
                          ("rns_24x2_16", lambda: extra_rns(ctx, stream)),
                          ("uneven_512", lambda: extra_uneven(ctx, stream)),
