@@ -141,7 +141,8 @@ struct PassJob {
 
 // Threads of every pass-kernel workgroup (a compile-time constant so that the
 // uniform index arithmetic folds).
-constexpr uint32_t kNT = 512;
+constexpr uint32_t kLogNT = 9;
+constexpr uint32_t kNT = 1u << kLogNT;
 
 // LDS addressing by byte offset.  The pass kernel declares no static LDS, so
 // its dynamic LDS starts at LDS address 0 (launch_pass_kernel checks the
@@ -497,7 +498,7 @@ __device__ __forceinline__ void store_loop_step(const PassJob& J, const Mod32& m
     using V = uint32_t;
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
     const uint32_t tid = threadIdx.x;
-    const uint32_t log_tpc = 9u - log_c;  // kNT = 2^9 threads, 2^log_tpc per column
+    const uint32_t log_tpc = kLogNT - log_c;  // 2^log_tpc threads per column
     const uint32_t cl = tid >> log_tpc, m = tid & ((1u << log_tpc) - 1u);
     const uint32_t cg = col0 + cl;
     const uint32_t kstep = 4u << log_tpc;

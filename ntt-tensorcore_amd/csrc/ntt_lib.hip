@@ -447,12 +447,12 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
     // c >> e_shift with C <= 2^e_shift) and the table is small.
     // The first pass of a P-pass plan (row-fast store) with 32-bit arithmetic
     // steps its factors down each column instead (column table), when every
-    // thread gets whole quads: log_c <= 9 and R >= 4 * 2^(9 - log_c).
+    // thread gets whole quads: log_c <= log2(threads) and R >= 4 * threads / 2^log_c.
     auto post = [&](PassSpec& s, uint64_t base, uint32_t mbits, uint64_t mult, bool first) {
         tb.pow_tables(s, base, mbits, mult);
         const uint32_t R = 1u << s.log_r;
-        if (first && P->npass >= 2 && arith == Arith::M32 && s.log_c <= 9) {
-            const uint32_t kstep = 4u << (9 - s.log_c);
+        if (first && P->npass >= 2 && arith == Arith::M32 && s.log_c <= kLogNT) {
+            const uint32_t kstep = 4u << (kLogNT - s.log_c);
             if (R >= kstep && (N >> s.log_r) <= kRowTableMax) tb.cols(s, base, (uint32_t)(N >> s.log_r), kstep);
             return;
         }
