@@ -169,6 +169,32 @@ def extra_reference_table(stream):
     return out
 
 
+def extra_polymul(stream):
+    """Negacyclic product a*b mod (x^n+1, BabyBear) on the uint64_t ABI
+    (ntt_c_negacyclic_multiply: two forward transforms and an inverse whose
+    first pass loads the pointwise product), against the same pipeline with the
+    product as its own kernel (forward, forward, ntt_c_poly_multiply, inverse).
+    Calls repeat on the same buffers (a and b stay residues mod q)."""
+    out = {}
+    for log_n in (12, 16, 20, 24):
+        n = 1 << log_n
+        cfg = nt.Config(n, BB)
+        a = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 1))
+        b = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 2))
+        r = nt.DeviceBuffer(n * 8)
+        fused = lambda: cfg.negacyclic_multiply(r, a, b, stream=stream)  # noqa: E731
+        split = lambda: (cfg.forward_negacyclic(a, stream), cfg.forward_negacyclic(b, stream),  # noqa: E731
+                         cfg.poly_multiply(r, a, b, stream=stream), cfg.inverse_negacyclic(r, stream))
+        reps = 20 if log_n < 24 else 5
+        t_f = float(np.median([timed(fused, reps, 2, stream) * 1e3 / reps for _ in range(3)]))
+        t_s = float(np.median([timed(split, reps, 2, stream) * 1e3 / reps for _ in range(3)]))
+        for x in (a, b, r):
+            x.free()
+        cfg.cleanup()
+        out[f"2^{log_n}"] = {"fused_us": t_f, "separate_pointwise_us": t_s, "speedup": t_s / t_f}
+    return out
+
+
 def extra_2_20(ctx, stream):
     out = {}
     for batch in (1, 16):
@@ -373,6 +399,7 @@ def main():
     if not args.no_extras:
         for name, fn in [("cyclic_4096", lambda: extra_latency_4096(ctx, stream)),
                          ("reference_table", lambda: extra_reference_table(stream)),
+                         ("negacyclic_multiply", lambda: extra_polymul(stream)),
                          ("negacyclic_2_20", lambda: extra_2_20(ctx, stream)),
                          ("rns_24x2_16", lambda: extra_rns(ctx, stream)),
                          ("uneven_512", lambda: extra_uneven(ctx, stream)),

@@ -68,6 +68,43 @@ __device__ __forceinline__ uint32_t inv_mod_2_32(uint32_t q) {
     return x;
 }
 
+// Montgomery product x * y * 2^-64 mod q (q < 2^62 odd, x, y < q), canonical;
+// qneg_inv = -q^-1 mod 2^64.  x*y + m*q is a multiple of 2^64 whose low
+// words sum to zero, carrying exactly when lo != 0; the quotient is < 2q.
+__device__ __forceinline__ uint64_t mont64(uint64_t x, uint64_t y, uint64_t q, uint64_t qneg_inv) {
+    const uint64_t lo = x * y, hi = __umul64hi(x, y);
+    const uint64_t m = lo * qneg_inv;
+    const uint64_t t = hi + __umul64hi(m, q) + (lo != 0);
+    return t >= q ? t - q : t;
+}
+
+// a*b mod q of canonical residues for the product fused into a pass's load
+// (JF_MUL_IN): a Montgomery product, then one by R^2 mod q to remove R^-1.
+template <class V>
+struct LoadMul;
+template <>
+struct LoadMul<uint32_t> {
+    uint32_t q, qinv, r2;
+    __device__ __forceinline__ explicit LoadMul(uint64_t q_, uint64_t r2_)
+        : q((uint32_t)q_), qinv(inv_mod_2_32((uint32_t)q_)), r2((uint32_t)r2_) {}
+    __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const {
+        return mont32(mont32(a, b, q, qinv), r2, q, qinv);
+    }
+};
+template <>
+struct LoadMul<uint64_t> {
+    uint64_t q, qneg_inv, r2;
+    __device__ __forceinline__ explicit LoadMul(uint64_t q_, uint64_t r2_) : q(q_), qneg_inv(0), r2(r2_) {
+        uint64_t x = q_;  // Newton from q (right to 3 bits): 5 steps reach 96 bits
+#pragma unroll
+        for (int i = 0; i < 5; i++) x *= 2u - q_ * x;
+        qneg_inv = 0 - x;
+    }
+    __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const {
+        return mont64(mont64(a, b, q, qneg_inv), r2, q, qneg_inv);
+    }
+};
+
 // q < 2^62: 2q < 2^63, Shoup with a 64-bit quotient.
 struct Mod64 {
     using V = uint64_t;
@@ -107,6 +144,8 @@ enum JobFlags : uint32_t {
     JF_POST_STEP = 1024u, // row-fast first-pass store with per-column stepped factors (tw_col)
     JF_LAZY_OUT = 2048u,  // row-table post may leave values in [0, 2q): the next pass's matrix-core
                           // round takes any 32-bit representative
+    JF_MUL_IN = 4096u,    // the load multiplies in[i] by in2[i] mod q (pointwise product fused into
+                          // the first pass of an inverse transform); inputs canonical
 };
 
 struct PassJob {
@@ -119,7 +158,9 @@ struct PassJob {
     const void* tw_row;   // row table: factor of (u, k) at [u*R + k], u = first column >> row_shift
     const void* tw_col;   // column table of JF_POST_STEP: 4 Shoup pairs per column
     const void* mz;       // JF_MFMA: matrix block of the radix-16 rounds (ntt_mf16.hpp)
+    const void* in2;      // JF_MUL_IN: second factor, same layout and storage as in
     uint64_t q;
+    uint64_t mul_r2;      // JF_MUL_IN: R^2 mod q, R = 2^(8*sizeof(V)) (undoes the Montgomery factor)
     uint64_t scale_w, scale_wp;  // POST_SCALE constant; with JF_POST_STEP, 2^32 mod q (Shoup pair)
     int64_t s_r, s_c;
     int64_t o_r, o_c, o_hs;
@@ -581,8 +622,9 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
 }
 
 // Tile load into LDS (row-major, swizzled): PER elements per thread and
-// sweep, PRED per-element bounds predicates, LR/LC as in store_loop.
-template <int PER, bool PRED, int LR, int LC, class Sin, class V>
+// sweep, PRED per-element bounds predicates, LR/LC as in store_loop; MUL
+// stores in[i]*in2[i] mod q (JF_MUL_IN).
+template <int PER, bool PRED, int LR, int LC, class Sin, class V, bool MUL = false>
 __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint32_t col0) {
     constexpr uint32_t VB = sizeof(V);
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
@@ -608,27 +650,34 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
     uint32_t e_x[PER];  // byte XOR offsets of the elements of a quad
 #pragma unroll
     for (int e = 0; e < PER; e++) e_x[e] = cf ? (uint32_t)e * VB : swz((uint32_t)e << log_c) * VB;
-    const Sin* __restrict__ in = reinterpret_cast<const Sin*>(J.in) + (int64_t)poly * J.in_pstride +
-                                 ((int64_t)col0 << sh_sc);
+    const int64_t base = (int64_t)poly * J.in_pstride + ((int64_t)col0 << sh_sc);
+    const Sin* __restrict__ in = reinterpret_cast<const Sin*>(J.in) + base;
+    const Sin* __restrict__ in2 = MUL ? reinterpret_cast<const Sin*>(J.in2) + base : nullptr;
+    auto fetch = [&](const Sin* __restrict__ p, bool ok, Sin* v) {
+        if (PER == 4) {
+            if (ok) load4<Sin>(p, v);
+            else {
+#pragma unroll
+                for (int e = 0; e < PER; e++) v[e] = 0;
+            }
+        } else {
+            v[0] = ok ? *p : (Sin)0;
+        }
+    };
+    const LoadMul<V> mul(MUL ? J.q : 3u, J.mul_r2);
     for (uint32_t bb = 0; bb < total; bb += kUnroll * sweep) {
         Sin v[kUnroll][PER];
+        Sin w[MUL ? kUnroll : 1][PER];
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
             const uint32_t ui = bb + (uint32_t)u * sweep;
             if (ui >= total) break;
             uint32_t ur, uc;
             rc(ui, ur, uc);
-            const Sin* __restrict__ p = in + (goff(ur, uc) + l_off);
+            const uint32_t off = goff(ur, uc) + l_off;
             const bool ok = !PRED || (lane_ok && col0 + (uc | lc) < ncols);
-            if (PER == 4) {
-                if (ok) load4<Sin>(p, v[u]);
-                else {
-#pragma unroll
-                    for (int e = 0; e < PER; e++) v[u][e] = 0;
-                }
-            } else {
-                v[u][0] = ok ? *p : (Sin)0;
-            }
+            fetch(in + off, ok, v[u]);
+            if (MUL) fetch(in2 + off, ok, w[MUL ? u : 0]);
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
@@ -639,7 +688,11 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
             rc(ui, ur, uc);
             const uint32_t s = l_slot ^ slot(ur, uc);
 #pragma unroll
-            for (int e = 0; e < PER; e++) lds_st<V>(s ^ e_x[e], (V)v[u][e]);
+            for (int e = 0; e < PER; e++) {
+                V x = (V)v[u][e];
+                if (MUL) x = mul(x, (V)w[MUL ? u : 0][e]);
+                lds_st<V>(s ^ e_x[e], x);
+            }
         }
     }
 }
@@ -766,6 +819,9 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
     const bool pred_l = !full || total < ((flags & JF_VEC4) ? 4u : 1u) * kNT;
     if (LR) {  // shape-specialised launches are full tiles with vector loads
         load_loop<4, false, LR, LC, Sin, V>(J, poly, col0);
+    } else if (flags & JF_MUL_IN) {  // pointwise product fused into the load (predicated: rare path)
+        if (flags & JF_VEC4) load_loop<4, true, LR, LC, Sin, V, true>(J, poly, col0);
+        else load_loop<1, true, LR, LC, Sin, V, true>(J, poly, col0);
     } else if (flags & JF_VEC4) {
         if (pred_l) load_loop<4, true, LR, LC, Sin, V>(J, poly, col0);
         else load_loop<4, false, LR, LC, Sin, V>(J, poly, col0);
@@ -815,13 +871,6 @@ __global__ void __launch_bounds__(NT) pass_kernel(const PassJob* __restrict__ jo
     run_pass<M, Sin, Sout, MAXM, P4, LR, LC>(J, tile - J.tile_begin);
 }
 
-// Persistent form of the shape-specialised pass kernel (32-bit coefficients
-// in and out, full tiles, vector loads and stores): workgroup b walks the
-// tiles xcd_remap(b + k*grid) (grid a multiple of 8, so every tile of a
-// workgroup lies in its XCD's contiguous range) and, once its current tile
-// is in LDS, issues the global loads of its next tile into registers, which
-// arrive while the current tile runs its rounds and its store.  Job tables
-// are staged when the job changes, a row table when the tile's row changes.
 // ---- small elementwise kernels ----
 
 // r = a*b mod q with q < 2^31 on uint64 storage (poly_mul_kernel semantics,
@@ -837,10 +886,9 @@ __global__ void pointwise_mul_kernel(S* r, const S* a, const S* b, uint64_t n, u
     }
 }
 
-// 62-bit pointwise product: 128-bit product reduced by Barrett with a
-// 128-bit reciprocal (mu_hi:mu_lo = floor(2^124/q)).
+// 62-bit pointwise product r = a*b mod q: mont64(mont64(a, b), 2^128 mod q).
 __global__ void pointwise_mul62_kernel(uint64_t* r, const uint64_t* a, const uint64_t* b, uint64_t n, uint64_t q,
-                                       uint64_t mu_lo, uint64_t mu_hi);
+                                       uint64_t qneg_inv, uint64_t r2);
 
 // In-place bit reversal of n = 2^log_n uint64 (reference bitrev semantics).
 __global__ void bitrev_kernel(uint64_t* d, uint64_t n, uint32_t log_n);

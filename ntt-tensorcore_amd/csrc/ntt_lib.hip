@@ -48,16 +48,9 @@ namespace nttk {
 
 __global__ void pointwise_mul62_kernel(uint64_t* r, const uint64_t* a, const uint64_t* b, uint64_t n, uint64_t q,
                                        uint64_t qneg_inv, uint64_t r2) {
-    // Montgomery with R = 2^64: mont(x,y) = x*y*R^-1 mod q; mont(mont(a,b), R^2) = a*b.
-    auto mont = [&](uint64_t x, uint64_t y) -> uint64_t {
-        const uint64_t lo = x * y, hi = __umul64hi(x, y);
-        const uint64_t m = lo * qneg_inv;
-        const uint64_t t = hi + __umul64hi(m, q) + (lo != 0);
-        return t >= q ? t - q : t;
-    };
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) r[i] = mont(mont(a[i], b[i]), r2);
+    for (; i < n; i += stride) r[i] = mont64(mont64(a[i], b[i], q, qneg_inv), r2, q, qneg_inv);
 }
 
 __global__ void bitrev_kernel(uint64_t* d, uint64_t n, uint32_t log_n) {
@@ -519,8 +512,16 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
     return NTT_OK;
 }
 
-// Scratch elements (arithmetic width) one polynomial needs for this plan.
-static uint64_t plan_scratch_elems(const Plan& P) { return P.npass >= 2 ? (1ull << P.log_n) : 0; }
+// Scratch elements (arithmetic width) one polynomial needs for this plan on
+// data stored as st.  With 32-bit arithmetic on uint64 data, plans of three or
+// more passes keep every intermediate in two 32-bit regions instead of
+// writing 8-byte words back into the data (make_jobs).
+static bool two_scratch(const Plan& P, Store st) {
+    return P.arith == Arith::M32 && st == Store::U64 && P.npass >= 3;
+}
+static uint64_t plan_scratch_elems(const Plan& P, Store st) {
+    return P.npass >= 2 ? (1ull << P.log_n) << (two_scratch(P, st) ? 1 : 0) : 0;
+}
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
@@ -538,7 +539,7 @@ static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma)
     const bool scf = J.o_c == 1 && J.o_lbits >= 2;
     if (scf) f |= JF_STORE_CFAST;
     const int64_t ib = (int64_t)sbytes(sin), ob = (int64_t)sbytes(sout);
-    bool vl = aligned16(J.in) && (J.in_pstride * ib) % 16 == 0;
+    bool vl = aligned16(J.in) && (!J.in2 || aligned16(J.in2)) && (J.in_pstride * ib) % 16 == 0;
     if (lcf) vl = vl && C >= 4 && (J.s_r % 4) == 0 && (J.ncols % 4) == 0;
     else vl = vl && J.s_r == 1 && R >= 4 && (J.s_c % 4) == 0;
     if (vl) f |= JF_VEC4;
@@ -578,8 +579,6 @@ static uint32_t job_lds(const PassJob& J, Arith ar) {
 // Launch kind of a job: which kernel instantiation runs it.
 static uint32_t job_kind(const PassJob& J) { return J.flags & (JF_MFMA | JF_POST4); }
 
-// Fills the PassJobs of `npoly` polynomials of plan P at data (stride n
-// elements), with workspace at `scratch` (stride n, arithmetic width).
 // log2 of every stride of a job.  make_jobs only makes strides that are powers
 // of two (N >> lr, 1, N, 2^ll, 2^(lr+ll)) or zero where the stride only
 // multiplies a term that is always zero; anything else is a bug in this file.
@@ -601,12 +600,18 @@ static void set_shifts(PassJob& J) {
     J.sh_ohs = stride_shift(J.o_hs);
 }
 
+// Fills the PassJobs of `npoly` polynomials of plan P at data (stride n
+// elements), with workspace at `scratch` (stride n, arithmetic width).  With
+// mul_a, the first pass reads mul_a[i]*mul_b[i] mod q (canonical residues,
+// same layout and storage as data) instead of data.
 static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, PassJob out[4],
-                      Store sin_out[4][2]) {
+                      Store sin_out[4][2], const void* mul_a = nullptr, const void* mul_b = nullptr) {
     const uint32_t log_n = P.log_n;
     const int64_t N = (int64_t)1 << log_n;
     const char* blk = (const char*)P.block;
     const Store mid = P.arith == Arith::M32 ? Store::U32 : Store::U64;
+    // second 32-bit scratch region (after npoly polynomials of the first)
+    void* const s2 = two_scratch(P, st) ? (char*)scratch + ((npoly << log_n) << 2) : nullptr;
     void* src = data;
     Store src_st = st;
     for (int i = 0; i < P.npass; i++) {
@@ -629,10 +634,19 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         J.e_a = s.e_a; J.e_b = s.e_b; J.e_d = s.e_d;
         J.e_shift = s.e_shift; J.e_mask = s.e_mask; J.lo_bits = s.lo_bits; J.mid_bits = s.mid_bits;
         const bool last = i == P.npass - 1;
-        void* dst = last ? data : (src == data ? scratch : data);
-        const Store dst_st = last ? st : (src == data ? mid : st);
+        void* dst;
+        Store dst_st;
+        if (last) dst = data, dst_st = st;
+        else if (s2) dst = src == scratch ? s2 : scratch, dst_st = mid;
+        else dst = src == data ? scratch : data, dst_st = src == data ? mid : st;
         J.in = src;
         J.out = dst;
+        if (i == 0 && mul_a) {
+            J.in = mul_a;
+            J.in2 = mul_b;
+            const u128 r = P.arith == Arith::M32 ? ((u128)1 << 32) % P.q : ((u128)1 << 64) % P.q;
+            J.mul_r2 = (uint64_t)(r * r % P.q);
+        }
         sin_out[i][0] = src_st;
         sin_out[i][1] = dst_st;
         if (P.npass == 1) {
@@ -660,6 +674,7 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         // a radix-16 MFMA round needs one full group of 16 units: >= 2^8 elements per tile
         finish_flags(J, P.arith, src_st, dst_st, s.mfma && J.log_r + J.log_c >= 8);
         if (s.cyc) J.flags |= JF_CT_CYC;
+        if (J.in2) J.flags |= JF_MUL_IN;
         // the stepped store needs staged power tables and row-fast vector stores
         if (J.tw_col && (J.flags & JF_POST_LDS) && !(J.flags & (JF_POST_ROW | JF_STORE_CFAST)) &&
             (J.flags & JF_VEC4_ST))
@@ -728,6 +743,7 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, co
 // in LDS, and a shape that has an instance.
 static uint32_t pass_shape_of(const PassJob& J, Arith ar, Store sin, Store sout) {
     if (ar != Arith::M32 || sin != Store::U32 || sout != Store::U32) return 0;
+    if (J.flags & JF_MUL_IN) return 0;
     if (!(J.flags & JF_VEC4) || !(J.flags & JF_VEC4_ST)) return 0;
     if (J.post == POST_FOURSTEP && !(J.flags & JF_POST_LDS)) return 0;
     if (J.ncols % (1u << J.log_c) != 0) return 0;
@@ -741,7 +757,6 @@ static uint32_t pass_shape_of(const PassJob& J, Arith ar, Store sin, Store sout)
     }
 }
 
-// Runs plan P on npoly polynomials at data (contiguous, stride n).
 // The shape-specialised kernel every job of a launch can use, or 0.
 static uint32_t group_shape(const PassJob* jobs, size_t n, Arith ar, Store sin, Store sout) {
     const uint32_t sh = n ? pass_shape_of(jobs[0], ar, sin, sout) : 0;
@@ -750,10 +765,13 @@ static uint32_t group_shape(const PassJob* jobs, size_t n, Arith ar, Store sin, 
     return sh;
 }
 
-static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, hipStream_t stream) {
+// Runs plan P on npoly polynomials at data (contiguous, stride n); with mul_a,
+// on mul_a*mul_b mod q elementwise, the result going to data (make_jobs).
+static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, hipStream_t stream,
+                           const void* mul_a = nullptr, const void* mul_b = nullptr) {
     PassJob jobs[4];
     Store io[4][2];
-    make_jobs(P, data, st, npoly, scratch, jobs, io);
+    make_jobs(P, data, st, npoly, scratch, jobs, io, mul_a, mul_b);
     for (int i = 0; i < P.npass; i++) {
         ntt_status s = launch_pass(P.arith, io[i][0], io[i][1], job_kind(jobs[i]), nullptr, 0, jobs[i],
                                    job_tiles(jobs[i]), job_lds(jobs[i], P.arith), stream,
@@ -886,7 +904,7 @@ extern "C" ntt_status ntt_c_cleanup(ntt_c_config* cfg) {
 }
 
 static ntt_status run_config(uint64_t* d, const ntt_c_config* cfg, bool nega, bool inv, uint64_t batch,
-                             void* stream) {
+                             void* stream, const uint64_t* mul_a = nullptr, const uint64_t* mul_b = nullptr) {
     if (!d || !cfg || !cfg->impl) return fail(NTT_ERR_INVALID_ARG, "null data or config");
     if (batch == 0) return NTT_OK;
     ConfigImpl* I = (ConfigImpl*)cfg->impl;
@@ -894,10 +912,10 @@ static ntt_status run_config(uint64_t* d, const ntt_c_config* cfg, bool nega, bo
     if (!I->plans[k]) return fail(I->plan_status[k], I->plan_error[k]);
     const Plan& P = *I->plans[k];
     void* scratch = nullptr;
-    const uint64_t sbytes = plan_scratch_elems(P) * batch * vbytes(P.arith);
+    const uint64_t sbytes = plan_scratch_elems(P, Store::U64) * batch * vbytes(P.arith);
     ntt_status st = scratch_for(I, (hipStream_t)stream, sbytes, &scratch);
     if (st != NTT_OK) return st;
-    return run_plan(P, d, Store::U64, batch, scratch, (hipStream_t)stream);
+    return run_plan(P, d, Store::U64, batch, scratch, (hipStream_t)stream, mul_a, mul_b);
 }
 
 static bool valid_variant(int v) { return v >= NTT_VARIANT_BASIC && v <= NTT_VARIANT_FULL_OPTIMIZED; }
@@ -950,10 +968,11 @@ extern "C" ntt_status ntt_c_poly_multiply(uint64_t* r, const uint64_t* a, const 
 extern "C" ntt_status ntt_c_negacyclic_multiply(uint64_t* r, uint64_t* a, uint64_t* b, const ntt_c_config* cfg,
                                                 void* stream) {
     if (!r || !a || !b || !cfg) return fail(NTT_ERR_INVALID_ARG, "negacyclic_multiply: null argument");
+    // The pointwise product is read by the inverse transform's first pass
+    // (JF_MUL_IN), so a*b is never written out.
     ntt_status st = ntt_c_forward_negacyclic(a, cfg, stream);
     if (st == NTT_OK) st = ntt_c_forward_negacyclic(b, cfg, stream);
-    if (st == NTT_OK) st = pointwise(r, a, b, cfg->n, cfg->q, (hipStream_t)stream);
-    if (st == NTT_OK) st = ntt_c_inverse_negacyclic(r, cfg, stream);
+    if (st == NTT_OK) st = run_config(r, cfg, true, true, 1, stream, a, b);
     return st;
 }
 
@@ -997,7 +1016,7 @@ extern "C" ntt_status ntt_c_forward_rowmaj(uint64_t* d, const ntt_c_config* cfg,
     const uint32_t n1 = 1u << l1, n2 = 1u << l2;
     const Plan& P = *I->plans[0];
     const uint64_t tbytes = cfg->n * sizeof(uint64_t);
-    const uint64_t sbytes = plan_scratch_elems(P) * vbytes(P.arith);
+    const uint64_t sbytes = plan_scratch_elems(P, Store::U64) * vbytes(P.arith);
     void* ws = nullptr;
     ntt_status st = scratch_for(I, (hipStream_t)stream, tbytes + sbytes, &ws);
     if (st != NTT_OK) return st;
@@ -1142,7 +1161,8 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
     for (size_t i = 0; i < count; i++) {
         ntt_status st = ctx_plan(ctx, descs[i], elems, &plans[i]);
         if (st != NTT_OK) return st;
-        scratch_bytes += ((plan_scratch_elems(*plans[i]) * vbytes(plans[i]->arith)) + 255) & ~255ull;
+        const Store sd = (descs[i].flags & NTT_FLAG_U64) ? Store::U64 : Store::U32;
+        scratch_bytes += ((plan_scratch_elems(*plans[i], sd) * vbytes(plans[i]->arith)) + 255) & ~255ull;
     }
     if (scratch_bytes) {
         hipError_t e = hipMalloc(&B->scratch, scratch_bytes);
@@ -1156,7 +1176,7 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
         void* scr = nullptr;
         if (P.npass >= 2) {
             scr = (char*)B->scratch + soff;
-            soff += ((plan_scratch_elems(P) * vbytes(P.arith)) + 255) & ~255ull;
+            soff += ((plan_scratch_elems(P, st) * vbytes(P.arith)) + 255) & ~255ull;
         }
         PassJob jobs[4];
         Store io[4][2];
@@ -1293,9 +1313,10 @@ static ntt_status transform_one(ntt_c_context* ctx, const ntt_c_desc* d, const S
     // The four-step workspace is a context-owned per-stream buffer (no
     // allocation once it is large enough).
     void* scr = nullptr;
-    st = ctx_scratch(ctx, (hipStream_t)stream, plan_scratch_elems(*P) * vbytes(P->arith), &scr);
+    const Store sd = (d->flags & NTT_FLAG_U64) ? Store::U64 : Store::U32;
+    st = ctx_scratch(ctx, (hipStream_t)stream, plan_scratch_elems(*P, sd) * vbytes(P->arith), &scr);
     if (st != NTT_OK) return st;
-    return run_plan(*P, d->data, (d->flags & NTT_FLAG_U64) ? Store::U64 : Store::U32, 1, scr, (hipStream_t)stream);
+    return run_plan(*P, d->data, sd, 1, scr, (hipStream_t)stream);
 }
 
 extern "C" ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* d, void* stream) {

@@ -179,6 +179,50 @@ def test_negacyclic_multiply_vs_schoolbook(nt, orc, cfgs, q, n):
     assert np.array_equal(dr.download(U64, n), orc.negacyclic_schoolbook(a, b, q))
 
 
+# The product runs fwd(a), fwd(b), then an inverse whose first pass loads
+# fwd(a)*fwd(b): sizes with one, two and three passes, 32- and 62-bit primes.
+@pytest.mark.parametrize("q,log_n", [(BB, 10), (BB, 14), (BB, 16), (BB, 20), (P30, 18), (P62, 11), (P62, 15)])
+@pytest.mark.parametrize("alias", [None, "a", "b"])
+def test_negacyclic_multiply_vs_oracle_pipeline(nt, orc, cfgs, q, log_n, alias):
+    n = 1 << log_n
+    c = cfgs(n, q)
+    a, b = orc.random_poly(n, q, 7 + log_n), orc.random_poly(n, q, 91 + log_n)
+    fa, fb = orc.ntt_negacyclic(a, q), orc.ntt_negacyclic(b, q)
+    want = orc.ntt_negacyclic(orc.pointwise_mul(fa, fb, q), q, inverse=True)
+    da, db = nt.DeviceBuffer.from_array(a), nt.DeviceBuffer.from_array(b)
+    dr = {"a": da, "b": db}.get(alias) or nt.DeviceBuffer(n * 8)
+    c.negacyclic_multiply(dr, da, db)
+    nt.device_sync()
+    assert np.array_equal(dr.download(U64, n), want)
+    if alias != "a":
+        assert np.array_equal(da.download(U64, n), fa)
+    if alias != "b":
+        assert np.array_equal(db.download(U64, n), fb)
+    for buf in {id(x): x for x in (da, db, dr)}.values():
+        buf.free()
+
+
+# Size-independent: multiplying by c*x^k shifts a by k places negacyclically
+# (coefficients pushed past x^(n-1) return negated) and scales by c.
+@pytest.mark.parametrize("q,log_n,k", [(BB, 22, 1), (BB, 24, (1 << 23) + 12345), (P62, 20, 777)])
+def test_negacyclic_multiply_by_monomial(nt, orc, cfgs, q, log_n, k):
+    n = 1 << log_n
+    a = orc.random_poly(n, q, 5)
+    b = np.zeros(n, dtype=U64)
+    cst = 3
+    b[k] = cst
+    da, db = nt.DeviceBuffer.from_array(a), nt.DeviceBuffer.from_array(b)
+    cfgs(n, q).negacyclic_multiply(da, da, db)
+    nt.device_sync()
+    got = da.download(U64, n)
+    da.free()
+    db.free()
+    # r[j] = c*a[j-k] for j >= k, -c*a[j-k+n] for j < k
+    sa = np.roll(a, k).astype(object) * cst % q
+    sa[:k] = (q - sa[:k]) % q
+    assert np.array_equal(got, sa.astype(U64))
+
+
 @pytest.mark.parametrize("log_n", [1, 6, 13, 20])
 def test_bit_reverse(nt, orc, log_n):
     n = 1 << log_n
