@@ -14,7 +14,9 @@
 // transforms down the columns and writes each tile transposed (contiguous
 // rows), middle passes regroup, the last pass writes natural-order output in
 // place.  The reference's driver (src/ntt_optimized.cu:515-603) makes
-// 1 + log2N%6 + log2N/6 HBM passes for the same result; this makes P <= 3.
+// 1 + log2N%6 + log2N/6 HBM passes for the same result; this makes P <= 4
+// (three at 2^24).  With 32-bit arithmetic the CT levels of a pass may run as
+// radix-16 rounds on the INT8 matrix cores instead (ntt_mf16.hpp).
 //
 // Arithmetic: canonical values in [0,q).  Products by a constant use Shoup's
 // precomputed quotient w' = floor(w*2^B/q) (B = 32 or 64), which returns a
@@ -391,7 +393,9 @@ __device__ __forceinline__ void store4<uint64_t>(uint64_t* p, const uint64_t v[4
     reinterpret_cast<ulonglong2*>(p)[1] = make_ulonglong2(v[2], v[3]);
 }
 
-// Loads per thread kept in flight before their LDS stores.
+// Vector loads (or stores) per thread kept in flight: 8 of 32-bit words.  The
+// load loop holds the same bytes, so fewer of 64-bit words or of two inputs
+// (each extra load register costs occupancy).
 constexpr int kUnroll = 8;
 
 // Natural-order store of a tile whose CT output sits at LDS row bitrev(k),
@@ -665,11 +669,12 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
         }
     };
     const LoadMul<V> mul(MUL ? J.q : 3u, J.mul_r2);
-    for (uint32_t bb = 0; bb < total; bb += kUnroll * sweep) {
-        Sin v[kUnroll][PER];
-        Sin w[MUL ? kUnroll : 1][PER];
+    constexpr int UN = kUnroll * 4 / (int)(sizeof(Sin) * (MUL ? 2 : 1));
+    for (uint32_t bb = 0; bb < total; bb += UN * sweep) {
+        Sin v[UN][PER];
+        Sin w[MUL ? UN : 1][PER];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
+        for (int u = 0; u < UN; u++) {
             const uint32_t ui = bb + (uint32_t)u * sweep;
             if (ui >= total) break;
             uint32_t ur, uc;
@@ -680,7 +685,7 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
             if (MUL) fetch(in2 + off, ok, w[MUL ? u : 0]);
         }
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
+        for (int u = 0; u < UN; u++) {
             const uint32_t ui = bb + (uint32_t)u * sweep;
             if (ui >= total) break;
             if (PRED && !lane_ok) continue;
@@ -705,11 +710,11 @@ struct Staged {
     LdsTw<V> zl, pl, pm, ph;
 };
 
-template <class M, bool P4, int LR>
+template <class M, bool P4, int LR, bool MF>
 __device__ __forceinline__ Staged<typename M::V> staged_of(const PassJob& J) {
     using V = typename M::V;
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r;
-    const bool mf = std::is_same<M, Mod32>::value && (J.flags & JF_MFMA);
+    constexpr bool mf = MF && std::is_same<M, Mod32>::value;
     uint32_t after = (1u << (log_r + J.log_c)) * (uint32_t)sizeof(V);
     Staged<V> t;
     t.zl = LdsTw<V>{after};
@@ -722,7 +727,7 @@ __device__ __forceinline__ Staged<typename M::V> staged_of(const PassJob& J) {
 
 // Copies the job's tables into LDS: everything (job tables and the row
 // table of column col0), or (row_only) just the row table.
-template <class M, bool P4, int LR>
+template <class M, bool P4, int LR, bool MF>
 __device__ __forceinline__ void stage_tables(const PassJob& J, const Staged<typename M::V>& t, uint32_t col0,
                                              bool row_only) {
     using V = typename M::V;
@@ -731,7 +736,7 @@ __device__ __forceinline__ void stage_tables(const PassJob& J, const Staged<type
     const uint32_t T = kNT, tid = threadIdx.x;
     const uint32_t flags = J.flags;
     if (!row_only && (flags & JF_Z_LDS)) {
-        const bool mf = std::is_same<M, Mod32>::value && (flags & JF_MFMA);
+        constexpr bool mf = MF && std::is_same<M, Mod32>::value;
         const TwP<V>* zsrc = mf ? reinterpret_cast<const TwP<V>*>(reinterpret_cast<const char*>(J.mz) + kMfTwOff)
                                 : reinterpret_cast<const TwP<V>*>(J.z);
         const uint32_t nz = mf ? mf_tw_entries(log_r) : R;
@@ -752,9 +757,9 @@ __device__ __forceinline__ void stage_tables(const PassJob& J, const Staged<type
     }
 }
 
-// The Cooley-Tukey levels of the tile in LDS (VALU rounds, or radix-16
+// The Cooley-Tukey levels of the tile in LDS (VALU rounds, or with MF radix-16
 // rounds on the matrix cores), then the store with the pass's post-multiply.
-template <class M, class Sout, int MAXM, bool P4, int LR, int LC>
+template <class M, class Sout, int MAXM, bool P4, int LR, int LC, bool MF>
 __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<typename M::V>& t, uint32_t poly,
                                              uint32_t col0, bool full) {
     using V = typename M::V;
@@ -764,19 +769,14 @@ __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<type
     const M mod{(decltype(M::q))J.q};
     const uint32_t flags = J.flags;
     const TwP<V>* zg = reinterpret_cast<const TwP<V>*>(J.z);
-    const bool mf = std::is_same<M, Mod32>::value && (flags & JF_MFMA);
     const bool cyc = flags & JF_CT_CYC;
-    if constexpr (std::is_same<M, Mod32>::value) {
-        if (mf) {
-            const TwP<V>* twg = reinterpret_cast<const TwP<V>*>(reinterpret_cast<const char*>(J.mz) + kMfTwOff);
-            const bool canon = J.post == POST_NONE;
-            MfMat mat;
-            mf_load_mat(J.mz, mat);
-            if (flags & JF_Z_LDS) mf_levels(mod, zg, mat, t.zl, log_r, log_c, cyc, canon);
-            else mf_levels(mod, zg, mat, twg, log_r, log_c, cyc, canon);
-        }
-    }
-    if (mf) {
+    if constexpr (MF && std::is_same<M, Mod32>::value) {
+        const TwP<V>* twg = reinterpret_cast<const TwP<V>*>(reinterpret_cast<const char*>(J.mz) + kMfTwOff);
+        const bool canon = J.post == POST_NONE;
+        MfMat mat;
+        mf_load_mat(J.mz, mat);
+        if (flags & JF_Z_LDS) mf_levels(mod, zg, mat, t.zl, log_r, log_c, cyc, canon);
+        else mf_levels(mod, zg, mat, twg, log_r, log_c, cyc, canon);
     } else if (LR) {
         if (flags & JF_Z_LDS) ct_levels_c<LR, LC, MAXM, 0, M>(t.zl, mod, cyc);
         else ct_levels_c<LR, LC, MAXM, 0, M>(zg, mod, cyc);
@@ -800,7 +800,7 @@ __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<type
     }
 }
 
-template <class M, class Sin, class Sout, int MAXM, bool P4, int LR, int LC>
+template <class M, class Sin, class Sout, int MAXM, bool P4, int LR, int LC, bool MF>
 __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job) {
     using V = typename M::V;
     const uint32_t poly = tile_in_job / J.tiles_per_poly;
@@ -811,8 +811,8 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
     const uint32_t total = 1u << (log_r + log_c);
     const bool full = col0 + C <= J.ncols;  // every column of the tile exists
     const uint32_t flags = J.flags;
-    const Staged<V> t = staged_of<M, P4, LR>(J);
-    stage_tables<M, P4, LR>(J, t, col0, false);
+    const Staged<V> t = staged_of<M, P4, LR, MF>(J);
+    stage_tables<M, P4, LR, MF>(J, t, col0, false);
 
     // ---- load ----
     // Predicates are needed for partial tiles and for tiles smaller than one sweep.
@@ -830,7 +830,7 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
         else load_loop<1, false, LR, LC, Sin, V>(J, poly, col0);
     }
     lds_barrier();
-    ct_and_store<M, Sout, MAXM, P4, LR, LC>(J, t, poly, col0, full);
+    ct_and_store<M, Sout, MAXM, P4, LR, LC, MF>(J, t, poly, col0, full);
 }
 
 // Workgroup index -> logical tile so that consecutive tiles run on the same
@@ -857,18 +857,22 @@ __device__ __forceinline__ uint32_t find_job(const PassJob* __restrict__ jobs, u
 // job per tile range from a device array sorted by tile_begin; one tile per
 // workgroup.
 // LR > 0: every job of the launch has tiles of 2^LR x 2^LC, vector loads and
-// stores, full tiles and (with P4) LDS post tables; see pass_shape_ok().
-template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC>
+// stores, full tiles and (with P4) LDS post tables (pass_shape_of in ntt_lib.hip).
+// MF: every job has JF_MFMA (32-bit arithmetic); without it none has.  Each
+// instance holds only one kind of round, so its registers are those of that
+// kind.  (Forcing 8 waves per SIMD on the matrix-core shapes, 4 workgroups
+// per CU, fits without scratch but measured ~2% slower at 2^24.)
+template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC, bool MF>
 __global__ void __launch_bounds__(NT) pass_kernel(const PassJob* __restrict__ jobs, uint32_t njobs, PassJob inl) {
     // The dynamic LDS (no static LDS: it starts at address 0) is addressed by
     // byte offset through lds_ld / lds_st.
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     if (njobs == 0) {
-        run_pass<M, Sin, Sout, MAXM, P4, LR, LC>(inl, tile);
+        run_pass<M, Sin, Sout, MAXM, P4, LR, LC, MF>(inl, tile);
         return;
     }
     const PassJob J = jobs[find_job(jobs, njobs, tile)];
-    run_pass<M, Sin, Sout, MAXM, P4, LR, LC>(J, tile - J.tile_begin);
+    run_pass<M, Sin, Sout, MAXM, P4, LR, LC, MF>(J, tile - J.tile_begin);
 }
 
 // ---- small elementwise kernels ----

@@ -13,12 +13,12 @@ namespace nttk {
 constexpr int kPassThreads = (int)kNT;
 constexpr uint32_t kPassLdsMax = 81920;  // LDS bytes a pass workgroup may use
 
-// Launches pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC> with one
-// workgroup per tile (LR = 0: the generic kernel).
-template <class M, class Sin, class Sout, bool P4, int LR = 0, int LC = 0>
+// Launches pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC, MF> with one
+// workgroup per tile (LR = 0: the generic kernel; MF: matrix-core rounds).
+template <class M, class Sin, class Sout, bool P4, int LR = 0, int LC = 0, bool MF = false>
 hipError_t launch_pass_kernel(const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t ntiles, uint32_t lds,
                               hipStream_t stream) {
-    auto fn = pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC>;
+    auto fn = pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC, MF>;
     static std::once_flag once;
     static hipError_t ready = hipSuccess;
     std::call_once(once, [&] {
@@ -34,11 +34,16 @@ hipError_t launch_pass_kernel(const PassJob* djobs, uint32_t njobs, const PassJo
     return hipGetLastError();
 }
 
+#define NTT_PASS_INSTANCE_1(EXT, M, SIN, SOUT, P4, MF)                                                         \
+    EXT template hipError_t launch_pass_kernel<M, SIN, SOUT, P4, 0, 0, MF>(const PassJob*, uint32_t,           \
+                                                                          const PassJob&, uint32_t, uint32_t,  \
+                                                                          hipStream_t);
+// VALU-round kernels; Mod32 also has the matrix-core-round (MF) kernels.
 #define NTT_PASS_INSTANCES(EXT, M, SIN, SOUT)                                                                  \
-    EXT template hipError_t launch_pass_kernel<M, SIN, SOUT, false>(const PassJob*, uint32_t, const PassJob&, \
-                                                                     uint32_t, uint32_t, hipStream_t);        \
-    EXT template hipError_t launch_pass_kernel<M, SIN, SOUT, true>(const PassJob*, uint32_t, const PassJob&,  \
-                                                                    uint32_t, uint32_t, hipStream_t);
+    NTT_PASS_INSTANCE_1(EXT, M, SIN, SOUT, false, false) NTT_PASS_INSTANCE_1(EXT, M, SIN, SOUT, true, false)
+#define NTT_PASS_INSTANCES_MF(EXT, SIN, SOUT)                                                                  \
+    NTT_PASS_INSTANCES(EXT, Mod32, SIN, SOUT)                                                                  \
+    NTT_PASS_INSTANCE_1(EXT, Mod32, SIN, SOUT, false, true) NTT_PASS_INSTANCE_1(EXT, Mod32, SIN, SOUT, true, true)
 
 // Tile shapes (log2 R, log2 C) with a specialised kernel for 32-bit
 // coefficients: radix 2^6..2^12 at tiles of 2^12, 2^13 and 2^14 elements
@@ -52,22 +57,20 @@ hipError_t launch_pass_kernel(const PassJob* djobs, uint32_t njobs, const PassJo
 #define NTT_SHAPES(X) \
     NTT_SHAPES_A(X) NTT_SHAPES_B(X) NTT_SHAPES_C(X) NTT_SHAPES_D(X) NTT_SHAPES_E(X) NTT_SHAPES_F(X)
 
-#define NTT_SHAPE_INSTANCE_EXTERN(LR, LC)                                                                       \
-    extern template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, false, LR, LC>(                    \
-        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);                             \
-    extern template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, true, LR, LC>(                     \
+#define NTT_SHAPE_INSTANCE_1(EXT, LR, LC, P4, MF)                                                               \
+    EXT template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, P4, LR, LC, MF>(                      \
         const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);
-#define NTT_SHAPE_INSTANCE(LR, LC)                                                                              \
-    template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, false, LR, LC>(                           \
-        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);                             \
-    template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, true, LR, LC>(                            \
-        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);
+#define NTT_SHAPE_INSTANCE_ALL(EXT, LR, LC)                                                                     \
+    NTT_SHAPE_INSTANCE_1(EXT, LR, LC, false, false) NTT_SHAPE_INSTANCE_1(EXT, LR, LC, true, false)              \
+    NTT_SHAPE_INSTANCE_1(EXT, LR, LC, false, true) NTT_SHAPE_INSTANCE_1(EXT, LR, LC, true, true)
+#define NTT_SHAPE_INSTANCE_EXTERN(LR, LC) NTT_SHAPE_INSTANCE_ALL(extern, LR, LC)
+#define NTT_SHAPE_INSTANCE(LR, LC) NTT_SHAPE_INSTANCE_ALL(, LR, LC)
 
 // Instantiated in pass_*.hip.
-NTT_PASS_INSTANCES(extern, Mod32, uint32_t, uint32_t)
-NTT_PASS_INSTANCES(extern, Mod32, uint32_t, uint64_t)
-NTT_PASS_INSTANCES(extern, Mod32, uint64_t, uint32_t)
-NTT_PASS_INSTANCES(extern, Mod32, uint64_t, uint64_t)
+NTT_PASS_INSTANCES_MF(extern, uint32_t, uint32_t)
+NTT_PASS_INSTANCES_MF(extern, uint32_t, uint64_t)
+NTT_PASS_INSTANCES_MF(extern, uint64_t, uint32_t)
+NTT_PASS_INSTANCES_MF(extern, uint64_t, uint64_t)
 NTT_PASS_INSTANCES(extern, Mod64, uint64_t, uint64_t)
 NTT_SHAPES(NTT_SHAPE_INSTANCE_EXTERN)
 

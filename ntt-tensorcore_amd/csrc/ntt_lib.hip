@@ -694,48 +694,64 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
 
 static uint32_t job_tiles(const PassJob& J) { return J.tiles_per_poly * (J.npoly ? J.npoly : 1); }
 
-// One launch of the pass kernel for (arith, input storage, output storage).
-template <bool P4>
-static ntt_status launch_valu(Arith ar, Store sin, Store sout, const PassJob* djobs, uint32_t njobs,
-                              const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream) {
+// One launch of the generic pass kernel for (arith, input storage, output
+// storage); MF: the jobs run matrix-core rounds (32-bit arithmetic only).
+template <bool P4, bool MF>
+static ntt_status launch_generic(Arith ar, Store sin, Store sout, const PassJob* djobs, uint32_t njobs,
+                                 const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream) {
     hipError_t e;
     if (ar == Arith::M32 && sin == Store::U32) {
-        e = sout == Store::U32 ? launch_pass_kernel<Mod32, uint32_t, uint32_t, P4>(djobs, njobs, inl, nblocks, lds, stream)
-                               : launch_pass_kernel<Mod32, uint32_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+        e = sout == Store::U32
+                ? launch_pass_kernel<Mod32, uint32_t, uint32_t, P4, 0, 0, MF>(djobs, njobs, inl, nblocks, lds, stream)
+                : launch_pass_kernel<Mod32, uint32_t, uint64_t, P4, 0, 0, MF>(djobs, njobs, inl, nblocks, lds, stream);
     } else if (ar == Arith::M32) {
-        e = sout == Store::U32 ? launch_pass_kernel<Mod32, uint64_t, uint32_t, P4>(djobs, njobs, inl, nblocks, lds, stream)
-                               : launch_pass_kernel<Mod32, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+        e = sout == Store::U32
+                ? launch_pass_kernel<Mod32, uint64_t, uint32_t, P4, 0, 0, MF>(djobs, njobs, inl, nblocks, lds, stream)
+                : launch_pass_kernel<Mod32, uint64_t, uint64_t, P4, 0, 0, MF>(djobs, njobs, inl, nblocks, lds, stream);
     } else {
+        if (MF) return fail(NTT_ERR_DEVICE, "internal: matrix-core rounds need 32-bit arithmetic");
         e = launch_pass_kernel<Mod64, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
     }
     HIP_TRY(e);
     return NTT_OK;
 }
 
+// Shape-specialised launch; false if the shape has no instance.
+template <bool P4, bool MF>
+static bool launch_shape(uint32_t shape, const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t nblocks,
+                         uint32_t lds, hipStream_t stream, hipError_t* e) {
+    switch (shape) {
+#define NTT_SHAPE_CASE(LR, LC)                                                                                   \
+    case (LR << 8) | LC:                                                                                          \
+        *e = launch_pass_kernel<Mod32, uint32_t, uint32_t, P4, LR, LC, MF>(djobs, njobs, inl, nblocks, lds, stream); \
+        return true;
+        NTT_SHAPES(NTT_SHAPE_CASE)
+#undef NTT_SHAPE_CASE
+        default: return false;
+    }
+}
+
 // One launch of the pass kernel for (arith, input storage, output storage,
-// kind = JF_MFMA / JF_POST4 bits of its jobs).
+// kind = JF_MFMA / JF_POST4 bits of its jobs, all alike).
 static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, const PassJob* djobs, uint32_t njobs,
                               const PassJob& inl, uint32_t nblocks, uint32_t lds, hipStream_t stream,
                               uint32_t shape = 0) {
     if (nblocks == 0) return NTT_OK;
+    const bool p4 = kind & JF_POST4, mf = kind & JF_MFMA;
     if (shape) {
-        const bool p4 = kind & JF_POST4;
-        hipError_t e = hipErrorInvalidValue;
-        switch (shape) {
-#define NTT_SHAPE_CASE(LR, LC)                                                                                 \
-    case (LR << 8) | LC:                                                                                        \
-        e = p4 ? launch_pass_kernel<Mod32, uint32_t, uint32_t, true, LR, LC>(djobs, njobs, inl, nblocks, lds, stream)  \
-               : launch_pass_kernel<Mod32, uint32_t, uint32_t, false, LR, LC>(djobs, njobs, inl, nblocks, lds, stream); \
-        break;
-            NTT_SHAPES(NTT_SHAPE_CASE)
-#undef NTT_SHAPE_CASE
-            default: return fail(NTT_ERR_DEVICE, "internal: no kernel for this tile shape");
-        }
+        hipError_t e = hipSuccess;
+        const bool found = p4 ? (mf ? launch_shape<true, true>(shape, djobs, njobs, inl, nblocks, lds, stream, &e)
+                                    : launch_shape<true, false>(shape, djobs, njobs, inl, nblocks, lds, stream, &e))
+                              : (mf ? launch_shape<false, true>(shape, djobs, njobs, inl, nblocks, lds, stream, &e)
+                                    : launch_shape<false, false>(shape, djobs, njobs, inl, nblocks, lds, stream, &e));
+        if (!found) return fail(NTT_ERR_DEVICE, "internal: no kernel for this tile shape");
         HIP_TRY(e);
         return NTT_OK;
     }
-    if (kind & JF_POST4) return launch_valu<true>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream);
-    return launch_valu<false>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream);
+    if (p4) return mf ? launch_generic<true, true>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream)
+                      : launch_generic<true, false>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream);
+    return mf ? launch_generic<false, true>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream)
+              : launch_generic<false, false>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream);
 }
 
 // Shape-specialised kernel for job J, as (log_r << 8) | log_c, or 0: 32-bit
