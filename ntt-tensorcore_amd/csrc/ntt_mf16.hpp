@@ -18,8 +18,8 @@
 //      G[p][(v, b)] = F[p][v] * 2^(8b) mod q (centred, G = sum_c 2^(8c) F_c),
 //   5. recombines A_0 + 2^8 A_1 + 2^16 (A_2 + 2^8 A_3): the high half by one
 //      signed Montgomery product with 2^16 * 2^32 (mod q), plus a constant
-//      L0 that makes the sum positive; the byte-flip correction
-//      128 * rowsum(G) is folded into plane 2's initial values,
+//      L0 that makes the sum positive (plane 0's initial values); the
+//      byte-flip correction 128 * rowsum(G) is folded into plane 2's,
 //   6. writes the 4 results back to the 4 words it read (the accumulator
 //      layout puts output rows 4g..4g+3 of unit n in lane (g, n)).
 // Bounds (q < 2^31): |F_c|, |digit| <= 128 over K = 64 give |A_c| <= 2^20,
@@ -86,46 +86,69 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
     const uint32_t lowmask = (1u << sh) - 1u;
     // row-major word of unit u's element 0: u's low sh bits stay, the rest moves up by 4
     auto word = [&](uint32_t u) -> uint32_t { return (u & lowmask) | ((u & ~lowmask) << 4); };
-    const uint32_t lslot = swz(word(n) | ((4u * g) << sh)) * 4u;
-    uint32_t ex[4];
+    // Lane (g, n) of group ub reads unit ub | n, elements v = 4g + e.  Every map
+    // here is GF(2)-linear and ub, n occupy disjoint bits, so an address is
+    // (lane part, kept in registers for the round) ^ (group part, a scalar).
+    // (The empty asm keeps the compiler from refactoring la ^ U back into
+    // (word ^ word) * 4, one more shift per access.)
+    uint32_t la[4];
 #pragma unroll
-    for (int e = 0; e < 4; e++) ex[e] = swz((uint32_t)e << sh) * 4u;
+    for (int e = 0; e < 4; e++) {
+        la[e] = swz(word(n) | ((4u * g + (uint32_t)e) << sh)) * 4u;
+        __asm__("" : "+v"(la[e]));
+    }
+    // twiddle index (((ub | n) >> sh) << 4) | 4g = lane part | (ub >> sh) << 4
+    const uint32_t lti = ((n >> sh) << 4) | (4u * g);
+    // with the table in LDS: byte address of the lane part
+    uint32_t lta = 0;
+    if constexpr (std::is_same<TP, LdsTw<uint32_t>>::value) {
+        lta = tw.off + lti * (uint32_t)sizeof(TwP<uint32_t>);
+        __asm__("" : "+v"(lta));
+    }
     const uint32_t qr = CANON == 2 ? (uint32_t)(0xFFFFFFFFu / q) : 0u;  // floor((2^32 - 1) / q)
     const mf_v4i zero = {0, 0, 0, 0};
+    // L0 enters through plane 0's accumulator: |sum| <= 2^20 and L0 < 2^31 - 2^20
+    const mf_v4i l0v = {(int)M.l0, (int)M.l0, (int)M.l0, (int)M.l0};
     for (uint32_t grp = wave; grp < ngroups; grp += kNT / 64u) {
         const uint32_t ub = grp << 4;
-        const uint32_t s = lslot ^ (swz(word(ub)) * 4u);
+        const uint32_t U = __builtin_amdgcn_readfirstlane(swz(word(ub)) * 4u);
         uint32_t x[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) x[e] = lds_ld<uint32_t>(s ^ ex[e]);
+        for (int e = 0; e < 4; e++) x[e] = lds_ld<uint32_t>(la[e] ^ U);
         if (TW) {
-            const uint32_t ti = (((ub | n) >> sh) << 4) | (4u * g);
+            const uint32_t tu = __builtin_amdgcn_readfirstlane((ub >> sh) << 4);
+            TwP<uint32_t> t[4];
+            if constexpr (std::is_same<TP, LdsTw<uint32_t>>::value) {
+                const LdsTw<uint32_t> tl{lta + tu * (uint32_t)sizeof(TwP<uint32_t>)};
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const TwP<uint32_t> t = tw[ti + (uint32_t)e];
-                x[e] = x[e] * t.w - __umulhi(x[e], t.wp) * q;  // [0, 2q)
+                for (int e = 0; e < 4; e++) t[e] = tl[(uint32_t)e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) t[e] = tw[lti + tu + (uint32_t)e];
             }
+#pragma unroll
+            for (int e = 0; e < 4; e++) x[e] = x[e] * t[e].w - __umulhi(x[e], t[e].wp) * q;  // [0, 2q)
         }
         mf_v4i b;
 #pragma unroll
         for (int e = 0; e < 4; e++) b[e] = (int)(x[e] ^ 0x80808080u);
-        const mf_v4i a0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(M.a[0], b, zero, 0, 0, 0);
+        const mf_v4i a0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(M.a[0], b, l0v, 0, 0, 0);
         const mf_v4i a1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(M.a[1], b, zero, 0, 0, 0);
         const mf_v4i a2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(M.a[2], b, M.h, 0, 0, 0);
         const mf_v4i a3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(M.a[3], b, zero, 0, 0, 0);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const uint32_t lo = (uint32_t)a0[e] + ((uint32_t)a1[e] << 8);
+            const uint32_t lo = (uint32_t)a0[e] + ((uint32_t)a1[e] << 8);  // includes L0
             const int32_t hi = (int32_t)((uint32_t)a2[e] + ((uint32_t)a3[e] << 8));
             const int32_t m = hi * (int32_t)M.cmq;
             const int32_t r = __mulhi(hi, (int32_t)M.cm) - __mulhi(m, (int32_t)q);  // hi * 2^16 mod q
-            uint32_t y = lo + (uint32_t)r + M.l0;
+            uint32_t y = lo + (uint32_t)r;
             if (CANON == 1) y = min(y, y - q);
             if (CANON == 2) {
                 y = y - __umulhi(y, qr) * q;  // [0, 2q)
                 y = min(y, y - q);
             }
-            lds_st<uint32_t>(s ^ ex[e], y);
+            lds_st<uint32_t>(la[e] ^ U, y);
         }
     }
 }
