@@ -102,9 +102,11 @@ ntt_status ntt_c_poly_multiply(uint64_t* d_result, const uint64_t* d_a, const ui
 
 /* Negacyclic product a*b mod (x^n+1, q) into d_result; d_a, d_b are
  * overwritten with their forward transforms (the pipeline of
- * tests/test_ntt.cu:177-240 on the negacyclic transform).  The pointwise
- * product is computed by the inverse transform's first pass as it loads its
- * tiles; d_result may alias d_a or d_b. */
+ * tests/test_ntt.cu:177-240 on the negacyclic transform).  The two forward
+ * transforms run as one batch of two polynomials, and the pointwise product
+ * is computed by the inverse transform's first pass as it loads its tiles.
+ * d_result may alias d_a or d_b; d_a == d_b squares (one forward transform);
+ * d_a and d_b otherwise must not overlap (NTT_ERR_INVALID_ARG). */
 ntt_status ntt_c_negacyclic_multiply(uint64_t* d_result, uint64_t* d_a, uint64_t* d_b, const ntt_c_config* cfg,
                                      void* stream);
 

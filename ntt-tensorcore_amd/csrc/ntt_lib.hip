@@ -601,13 +601,16 @@ static void set_shifts(PassJob& J) {
 }
 
 // Fills the PassJobs of `npoly` polynomials of plan P at data (stride n
-// elements), with workspace at `scratch` (stride n, arithmetic width).  With
-// mul_a, the first pass reads mul_a[i]*mul_b[i] mod q (canonical residues,
-// same layout and storage as data) instead of data.
+// elements, or dstride with two or more passes), with workspace at `scratch`
+// (stride n, arithmetic width).  With mul_a, the first pass reads
+// mul_a[i]*mul_b[i] mod q (canonical residues, same layout and storage as
+// data) instead of data.
 static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, PassJob out[4],
-                      Store sin_out[4][2], const void* mul_a = nullptr, const void* mul_b = nullptr) {
+                      Store sin_out[4][2], const void* mul_a = nullptr, const void* mul_b = nullptr,
+                      int64_t dstride = 0) {
     const uint32_t log_n = P.log_n;
     const int64_t N = (int64_t)1 << log_n;
+    if (dstride == 0) dstride = N;
     const char* blk = (const char*)P.block;
     const Store mid = P.arith == Arith::M32 ? Store::U32 : Store::U64;
     // second 32-bit scratch region (after npoly polynomials of the first)
@@ -668,7 +671,8 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
             J.ncols = (uint32_t)(N >> lr);
             J.tiles_per_poly = J.ncols >> J.log_c;
             J.npoly = (uint32_t)npoly;
-            J.in_pstride = J.out_pstride = N;
+            J.in_pstride = src == data ? dstride : N;
+            J.out_pstride = dst == data ? dstride : N;
         }
         set_shifts(J);
         // a radix-16 MFMA round needs one full group of 16 units: >= 2^8 elements per tile
@@ -784,10 +788,10 @@ static uint32_t group_shape(const PassJob* jobs, size_t n, Arith ar, Store sin, 
 // Runs plan P on npoly polynomials at data (contiguous, stride n); with mul_a,
 // on mul_a*mul_b mod q elementwise, the result going to data (make_jobs).
 static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, hipStream_t stream,
-                           const void* mul_a = nullptr, const void* mul_b = nullptr) {
+                           const void* mul_a = nullptr, const void* mul_b = nullptr, int64_t dstride = 0) {
     PassJob jobs[4];
     Store io[4][2];
-    make_jobs(P, data, st, npoly, scratch, jobs, io, mul_a, mul_b);
+    make_jobs(P, data, st, npoly, scratch, jobs, io, mul_a, mul_b, dstride);
     for (int i = 0; i < P.npass; i++) {
         ntt_status s = launch_pass(P.arith, io[i][0], io[i][1], job_kind(jobs[i]), nullptr, 0, jobs[i],
                                    job_tiles(jobs[i]), job_lds(jobs[i], P.arith), stream,
@@ -984,10 +988,28 @@ extern "C" ntt_status ntt_c_poly_multiply(uint64_t* r, const uint64_t* a, const 
 extern "C" ntt_status ntt_c_negacyclic_multiply(uint64_t* r, uint64_t* a, uint64_t* b, const ntt_c_config* cfg,
                                                 void* stream) {
     if (!r || !a || !b || !cfg) return fail(NTT_ERR_INVALID_ARG, "negacyclic_multiply: null argument");
+    if (!cfg->impl) return fail(NTT_ERR_INVALID_ARG, "negacyclic_multiply: null config");
+    ConfigImpl* I = (ConfigImpl*)cfg->impl;
+    if (!I->plans[2]) return fail(I->plan_status[2], I->plan_error[2]);
+    const Plan& P = *I->plans[2];
+    const int64_t d = b - a;
+    ntt_status st = NTT_OK;
+    if (d == 0) {  // a squaring: one forward transform
+        st = ntt_c_forward_negacyclic(a, cfg, stream);
+    } else if ((uint64_t)(d < 0 ? -d : d) < cfg->n) {
+        return fail(NTT_ERR_INVALID_ARG, "negacyclic_multiply: a and b overlap");
+    } else if (P.npass >= 2) {
+        // both forward transforms in one launch per pass: a batch of two
+        // polynomials whose stride is b - a
+        void* scratch = nullptr;
+        st = scratch_for(I, (hipStream_t)stream, plan_scratch_elems(P, Store::U64) * 2 * vbytes(P.arith), &scratch);
+        if (st == NTT_OK) st = run_plan(P, a, Store::U64, 2, scratch, (hipStream_t)stream, nullptr, nullptr, d);
+    } else {
+        st = ntt_c_forward_negacyclic(a, cfg, stream);
+        if (st == NTT_OK) st = ntt_c_forward_negacyclic(b, cfg, stream);
+    }
     // The pointwise product is read by the inverse transform's first pass
     // (JF_MUL_IN), so a*b is never written out.
-    ntt_status st = ntt_c_forward_negacyclic(a, cfg, stream);
-    if (st == NTT_OK) st = ntt_c_forward_negacyclic(b, cfg, stream);
     if (st == NTT_OK) st = run_config(r, cfg, true, true, 1, stream, a, b);
     return st;
 }

@@ -202,6 +202,65 @@ def test_negacyclic_multiply_vs_oracle_pipeline(nt, orc, cfgs, q, log_n, alias):
         buf.free()
 
 
+@pytest.mark.parametrize("q,log_n", [(BB, 10), (BB, 16), (P62, 12)])
+def test_negacyclic_square(nt, orc, cfgs, q, log_n):
+    # d_a == d_b: one forward transform, then a*a
+    n = 1 << log_n
+    a = orc.random_poly(n, q, 33 + log_n)
+    fa = orc.ntt_negacyclic(a, q)
+    want = orc.ntt_negacyclic(orc.pointwise_mul(fa, fa, q), q, inverse=True)
+    if log_n <= 12:
+        assert np.array_equal(want, orc.negacyclic_schoolbook(a, a, q))
+    da, dr = nt.DeviceBuffer.from_array(a), nt.DeviceBuffer(n * 8)
+    cfgs(n, q).negacyclic_multiply(dr, da, da)
+    nt.device_sync()
+    assert np.array_equal(dr.download(U64, n), want)
+    assert np.array_equal(da.download(U64, n), fa)
+    da.free()
+    dr.free()
+
+
+# a and b inside one allocation: the two forward transforms run as a batch
+# whose polynomial stride is b - a (even, odd or negative, in elements).
+@pytest.mark.parametrize("q,log_n", [(BB, 14), (BB, 20), (P62, 13)])
+@pytest.mark.parametrize("gap", [0, 1, 6, -3])
+def test_negacyclic_multiply_strides(nt, orc, cfgs, q, log_n, gap):
+    n = 1 << log_n
+    a, b = orc.random_poly(n, q, 5 + log_n), orc.random_poly(n, q, 8 + log_n)
+    fa, fb = orc.ntt_negacyclic(a, q), orc.ntt_negacyclic(b, q)
+    want = orc.ntt_negacyclic(orc.pointwise_mul(fa, fb, q), q, inverse=True)
+    oa, ob = (0, n + gap) if gap >= 0 else (n - gap, 0)
+    host = np.zeros(2 * n + abs(gap), dtype=U64)
+    host[oa:oa + n], host[ob:ob + n] = a, b
+    buf = nt.DeviceBuffer.from_array(host)
+    dr = nt.DeviceBuffer(n * 8)
+
+    class At:
+        def __init__(self, off):
+            self.ptr = buf.ptr + off * 8
+    cfgs(n, q).negacyclic_multiply(dr, At(oa), At(ob))
+    nt.device_sync()
+    assert np.array_equal(dr.download(U64, n), want)
+    got = buf.download(U64, host.size)
+    assert np.array_equal(got[oa:oa + n], fa) and np.array_equal(got[ob:ob + n], fb)
+    buf.free()
+    dr.free()
+
+
+def test_negacyclic_multiply_rejects_overlap(nt, orc, cfgs):
+    n = 1 << 12
+    buf = nt.DeviceBuffer.from_array(orc.random_poly(2 * n, BB, 3))
+    dr = nt.DeviceBuffer(n * 8)
+
+    class View:  # b = a + 1000 elements: the two inputs overlap
+        ptr = buf.ptr + 1000 * 8
+    with pytest.raises(nt.NttError):
+        cfgs(n, BB).negacyclic_multiply(dr, buf, View())
+    nt.device_sync()
+    buf.free()
+    dr.free()
+
+
 # Size-independent: multiplying by c*x^k shifts a by k places negacyclically
 # (coefficients pushed past x^(n-1) return negated) and scales by c.
 @pytest.mark.parametrize("q,log_n,k", [(BB, 22, 1), (BB, 24, (1 << 23) + 12345), (P62, 20, 777)])
