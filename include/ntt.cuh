@@ -88,6 +88,12 @@ inline void ntt_forward_negacyclic(uint64_t* d, const NTTConfig* c, hipStream_t 
 inline void ntt_inverse_negacyclic(uint64_t* d, const NTTConfig* c, hipStream_t s = 0) {
     check_(ntt_c_inverse_negacyclic(d, c, s), "ntt_inverse_negacyclic");
 }
+// The product pipeline of reference tests/test_ntt.cu:177-240 (forward
+// negacyclic a and b, poly_multiply_ntt, inverse negacyclic) as one call: r =
+// a*b mod (x^n+1, q); a and b are left holding their forward transforms.
+inline void ntt_negacyclic_multiply(uint64_t* r, uint64_t* a, uint64_t* b, const NTTConfig* c, hipStream_t s = 0) {
+    check_(ntt_c_negacyclic_multiply(r, a, b, c, s), "ntt_negacyclic_multiply");
+}
 
 // reference :63-67 / src/ntt_optimized.cu:922-965
 inline void ntt_forward_rowmaj(uint64_t* d, const NTTConfig* c, hipStream_t s = 0) {

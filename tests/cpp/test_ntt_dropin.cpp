@@ -68,7 +68,7 @@ static bool roundtrip_and_host(uint64_t n, NTTVariant v, bool vs_host) {
     return ok;
 }
 
-static bool negacyclic_product(uint64_t n) {
+static bool negacyclic_product(uint64_t n, bool fused) {
     NTTConfig* c = ntt_init(n, DEFAULT_PRIME);
     auto a = random_polynomial(n, c->q, 42), b = random_polynomial(n, c->q, 123);
     std::vector<uint64_t> ref(n, 0);
@@ -84,10 +84,14 @@ static bool negacyclic_product(uint64_t n) {
     HCHECK(hipMalloc(&dr, n * 8));
     HCHECK(hipMemcpy(da, a.data(), n * 8, hipMemcpyHostToDevice));
     HCHECK(hipMemcpy(db, b.data(), n * 8, hipMemcpyHostToDevice));
-    ntt_forward_negacyclic(da, c);
-    ntt_forward_negacyclic(db, c);
-    poly_multiply_ntt(dr, da, db, c);
-    ntt_inverse_negacyclic(dr, c);
+    if (fused) {
+        ntt_negacyclic_multiply(dr, da, db, c);
+    } else {
+        ntt_forward_negacyclic(da, c);
+        ntt_forward_negacyclic(db, c);
+        poly_multiply_ntt(dr, da, db, c);
+        ntt_inverse_negacyclic(dr, c);
+    }
     std::vector<uint64_t> r(n);
     HCHECK(hipMemcpy(r.data(), dr, n * 8, hipMemcpyDeviceToHost));
     HCHECK(hipFree(da));
@@ -107,7 +111,8 @@ int main() {
     for (uint64_t n : {64ull, 256ull, 1024ull, 4096ull}) rec(roundtrip_and_host(n, NTTVariant::BASIC, true), "Basic roundtrip + vs host", n);
     for (uint64_t n : {4096ull, 8192ull, 16384ull}) rec(roundtrip_and_host(n, NTTVariant::FOUR_STEP, true), "FourStep roundtrip + vs host", n);
     for (uint64_t n = 4096; n <= (1ull << 20); n <<= 1) rec(roundtrip_and_host(n, NTTVariant::FULL_OPTIMIZED, n <= 65536), "Optimized roundtrip (+ vs host)", n);
-    for (uint64_t n : {64ull, 1024ull}) rec(negacyclic_product(n), "Negacyclic product vs schoolbook", n);
+    for (uint64_t n : {64ull, 1024ull}) rec(negacyclic_product(n, false), "Negacyclic product vs schoolbook", n);
+    for (uint64_t n : {64ull, 4096ull}) rec(negacyclic_product(n, true), "Fused negacyclic product", n);
     std::printf("Passed: %d / %d\n", passed, total);
     return passed == total ? 0 : 1;
 }
