@@ -113,6 +113,15 @@ ntt_status ntt_c_negacyclic_multiply(uint64_t* d_result, uint64_t* d_a, uint64_t
 /* bit_reverse_permute, reference src/ntt_basic.cu:259-265 (include/ntt.cuh:82). */
 ntt_status ntt_c_bit_reverse(uint64_t* d_data, uint64_t n, void* stream);
 
+/* Coefficient width conversion between the reference ABI's uint64_t arrays
+ * and this library's 32-bit storage (NTT_FLAG_U64 off), device to device on
+ * the stream.  No reference counterpart: the reference stores only uint64_t
+ * (include/ntt.cuh:35-85).  u64_to_u32 writes src[i] mod q (q < 2^32), so any
+ * 64-bit input becomes a canonical residue; u32_to_u64 zero-extends.  Both
+ * need 16-byte-aligned, non-overlapping buffers (NTT_ERR_INVALID_ARG). */
+ntt_status ntt_c_u64_to_u32(uint32_t* d_dst, const uint64_t* d_src, uint64_t n, uint64_t q, void* stream);
+ntt_status ntt_c_u32_to_u64(uint64_t* d_dst, const uint32_t* d_src, uint64_t n, void* stream);
+
 /* ntt_forward_rowmaj, reference src/ntt_optimized.cu:940-965 (include/ntt.cuh:63):
  * input and output in the pre-transposed layout data[b*N1 + a] = canon[a*N2 + b]
  * with N1 = 2^ceil(log n / 2), N2 = n / N1 (N1 = N2 = 64 at n = 4096, as in the

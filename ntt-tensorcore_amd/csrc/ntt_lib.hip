@@ -81,6 +81,37 @@ __global__ void transpose_u64_kernel(uint64_t* out, const uint64_t* in, uint32_t
     }
 }
 
+// Coefficient width conversion between the reference's uint64_t arrays and
+// the native 32-bit storage, 4 elements per thread and step (16-byte aligned
+// bulk, scalar tail).  Narrowing reduces mod q (q < 2^32) by Barrett with
+// mu = floor(2^64 / q): x - hi(x*mu)*q lies in [0, 2q).
+__global__ void narrow_u64_kernel(uint32_t* dst, const uint64_t* src, uint64_t n, uint64_t q, uint64_t mu) {
+    auto red = [&](uint64_t x) -> uint32_t {
+        const uint64_t t = x - __umul64hi(x, mu) * q;
+        return (uint32_t)(t >= q ? t - q : t);
+    };
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t n4 = n >> 2;
+    for (uint64_t i = tid; i < n4; i += stride) {
+        const ulonglong2 a = reinterpret_cast<const ulonglong2*>(src)[2 * i];
+        const ulonglong2 b = reinterpret_cast<const ulonglong2*>(src)[2 * i + 1];
+        reinterpret_cast<uint4*>(dst)[i] = make_uint4(red(a.x), red(a.y), red(b.x), red(b.y));
+    }
+    for (uint64_t i = 4 * n4 + tid; i < n; i += stride) dst[i] = red(src[i]);
+}
+__global__ void widen_u32_kernel(uint64_t* dst, const uint32_t* src, uint64_t n) {
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t n4 = n >> 2;
+    for (uint64_t i = tid; i < n4; i += stride) {
+        const uint4 a = reinterpret_cast<const uint4*>(src)[i];
+        reinterpret_cast<ulonglong2*>(dst)[2 * i] = make_ulonglong2(a.x, a.y);
+        reinterpret_cast<ulonglong2*>(dst)[2 * i + 1] = make_ulonglong2(a.z, a.w);
+    }
+    for (uint64_t i = 4 * n4 + tid; i < n; i += stride) dst[i] = src[i];
+}
+
 // out[k] = lo[k & mask] * hi[k >> bits] (Shoup on hi): powers omega^k from two tables.
 template <class M>
 __global__ void powers_kernel(uint64_t* out, uint64_t n, const TwP<typename M::V>* lo, const TwP<typename M::V>* hi,
@@ -1019,6 +1050,32 @@ extern "C" ntt_status ntt_c_bit_reverse(uint64_t* d, uint64_t n, void* stream) {
     const uint32_t log_n = (uint32_t)__builtin_ctzll(n);
     const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(bitrev_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, d, n, log_n);
+    HIP_TRY(hipGetLastError());
+    return NTT_OK;
+}
+
+// Grid of a 4-elements-per-thread elementwise pass over n elements.
+static unsigned elementwise_blocks(uint64_t n) { return (unsigned)std::min<uint64_t>((n / 4 + 255) / 256 + 1, 8192); }
+
+extern "C" ntt_status ntt_c_u64_to_u32(uint32_t* dst, const uint64_t* src, uint64_t n, uint64_t q, void* stream) {
+    if (!dst || !src || q < 2 || q >= (1ull << 32)) return fail(NTT_ERR_INVALID_ARG, "u64_to_u32: bad arguments");
+    if (((uintptr_t)dst & 15u) || ((uintptr_t)src & 15u)) return fail(NTT_ERR_INVALID_ARG, "u64_to_u32: pointers must be 16-byte aligned");
+    const uint64_t lo = (uint64_t)(uintptr_t)src, hi = (uint64_t)(uintptr_t)dst;
+    if (n && lo < hi + n * 4 && hi < lo + n * 8) return fail(NTT_ERR_INVALID_ARG, "u64_to_u32: dst overlaps src");
+    if (n == 0) return NTT_OK;
+    hipLaunchKernelGGL(narrow_u64_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, (hipStream_t)stream, dst, src, n, q,
+                       (uint64_t)(((u128)1 << 64) / q));
+    HIP_TRY(hipGetLastError());
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_u32_to_u64(uint64_t* dst, const uint32_t* src, uint64_t n, void* stream) {
+    if (!dst || !src) return fail(NTT_ERR_INVALID_ARG, "u32_to_u64: null argument");
+    if (((uintptr_t)dst & 15u) || ((uintptr_t)src & 15u)) return fail(NTT_ERR_INVALID_ARG, "u32_to_u64: pointers must be 16-byte aligned");
+    const uint64_t lo = (uint64_t)(uintptr_t)src, hi = (uint64_t)(uintptr_t)dst;
+    if (n && lo < hi + n * 8 && hi < lo + n * 4) return fail(NTT_ERR_INVALID_ARG, "u32_to_u64: dst overlaps src");
+    if (n == 0) return NTT_OK;
+    hipLaunchKernelGGL(widen_u32_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, (hipStream_t)stream, dst, src, n);
     HIP_TRY(hipGetLastError());
     return NTT_OK;
 }

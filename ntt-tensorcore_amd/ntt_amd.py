@@ -77,6 +77,8 @@ def lib():
         "ntt_c_poly_multiply": [vp, vp, vp, P(ConfigStruct), vp],
         "ntt_c_negacyclic_multiply": [vp, vp, vp, P(ConfigStruct), vp],
         "ntt_c_bit_reverse": [vp, u64, vp],
+        "ntt_c_u64_to_u32": [vp, vp, u64, u64, vp],
+        "ntt_c_u32_to_u64": [vp, vp, u64, vp],
         "ntt_c_forward_rowmaj": [vp, P(ConfigStruct), vp],
         "ntt_c_rowmaj_to_pretransposed": [vp, vp, i32, i32, vp],
         "ntt_c_rowmaj_from_pretransposed": [vp, vp, i32, i32, vp],
@@ -296,6 +298,16 @@ def ntt_host(data, params, inverse=False):
 
 def bit_reverse(buf, n, stream=None):
     check(lib().ntt_c_bit_reverse(buf.ptr, n, _sp(stream)))
+
+
+def u64_to_u32(dst, src, n, q, stream=None):
+    """dst (uint32) = src (uint64) mod q, device to device."""
+    check(lib().ntt_c_u64_to_u32(dst.ptr, src.ptr, n, q, _sp(stream)))
+
+
+def u32_to_u64(dst, src, n, stream=None):
+    """dst (uint64) = src (uint32) zero-extended, device to device."""
+    check(lib().ntt_c_u32_to_u64(dst.ptr, src.ptr, n, _sp(stream)))
 
 
 def rowmaj_to_pretransposed(dst, src, n1, n2, stream=None):
