@@ -244,6 +244,36 @@ def extra_rns(ctx, stream):
             "GB_s": 2 * n * 4 / t / 1e9, "frac": 2 * n * 4 / t / 1e9 / HBM_PEAK_GBS}
 
 
+def extra_rns_kernels(stream):
+    """RNS kernels around the transform on the 24 x 2^16 limb set: fast base
+    conversion of 24 limbs to 24 other primes (ntt_c_bconv; 576 multiply-adds
+    and 192 HBM bytes per coefficient) and a 3-term per-limb multiply-accumulate
+    (ntt_c_pmac, dnum = 3, accumulating: 32 HBM bytes per output residue)."""
+    n, L = 1 << 16, 24
+    both = W.ntt_primes(1 << 31, 17, 2 * L)
+    q, p = both[:L], both[L:]
+    rng = np.random.default_rng(7)
+    x = np.stack([rng.integers(0, v, size=n, dtype=np.uint64) for v in q]).astype(np.uint32)
+    din = nt.DeviceBuffer.from_array(x)
+    dout = nt.DeviceBuffer(L * n * 4)
+    bc = nt.Bconv(q, p)
+    t_b = timed(lambda: bc.run(dout, din, n, stream), 50, 5, stream) / 50 / 1e3
+    dnum = 3
+    a = nt.DeviceBuffer.from_array(np.concatenate([x] * dnum))
+    b = nt.DeviceBuffer.from_array(np.concatenate([x[::-1]] * dnum))
+    pm = nt.Pmac(q)
+    t_p = timed(lambda: pm.run(dout, a, b, dnum, n, accumulate=True, stream=stream), 50, 5, stream) / 50 / 1e3
+    for o in (bc, pm):
+        o.destroy()
+    for buf in (din, dout, a, b):
+        buf.free()
+    bytes_b, bytes_p = n * 4 * (2 * L), n * L * 4 * (2 * dnum + 2)
+    return {"bconv": {"L": L, "K": L, "n": n, "us": t_b * 1e6, "Gmac_s": n * L * L / t_b / 1e9,
+                      "GB_s": bytes_b / t_b / 1e9, "frac": bytes_b / t_b / 1e9 / HBM_PEAK_GBS},
+            "pmac": {"dnum": dnum, "L": L, "n": n, "us": t_p * 1e6, "GB_s": bytes_p / t_p / 1e9,
+                     "frac": bytes_p / t_p / 1e9 / HBM_PEAK_GBS}}
+
+
 def extra_uneven(ctx, stream):
     jobs = W.uneven_batch()
     buf, offs = _pack(jobs, np.uint32)
@@ -402,6 +432,7 @@ def main():
                          ("negacyclic_multiply", lambda: extra_polymul(stream)),
                          ("negacyclic_2_20", lambda: extra_2_20(ctx, stream)),
                          ("rns_24x2_16", lambda: extra_rns(ctx, stream)),
+                         ("rns_kernels", lambda: extra_rns_kernels(stream)),
                          ("uneven_512", lambda: extra_uneven(ctx, stream)),
                          ("u64_abi", lambda: extra_u64_abi(args, stream)),
                          ("klimb", lambda: extra_klimb(stream)),

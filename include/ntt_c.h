@@ -270,6 +270,33 @@ ntt_status ntt_c_arb_pointwise_mul(uint32_t* d_a, const uint32_t* d_b, uint32_t 
 ntt_status ntt_c_arb_scale(uint32_t* d_a, uint32_t n, const uint32_t* scalar_limbs, const uint32_t* prime_limbs,
                            uint32_t np, uint32_t K, void* stream);
 
+/* ---------------- RNS kernels around the transform.  No reference
+ * counterpart: the reference names them as absent (src/ntt_optimized.cu:510-513,
+ * "BConv/IP kernels (none exist)", "no PMAC in this codebase").  Residues are
+ * 32-bit, limb-major (limb i of n coefficients at d[i*n .. i*n + n)); moduli
+ * are odd primes below 2^31. ---------------- */
+typedef struct ntt_c_bconv_plan ntt_c_bconv_plan;
+typedef struct ntt_c_pmac_plan ntt_c_pmac_plan;
+
+/* Fast base conversion from the L moduli q_from (distinct, L <= 32) to the K
+ * moduli p_to (K <= 64): y_j = sum_i [x_i (Q/q_i)^-1]_{q_i} (Q/q_i) mod p_j,
+ * i.e. (X + u Q) mod p_j for the CRT value X of x and some 0 <= u < L.  The
+ * plan holds its constant tables on the device. */
+ntt_status ntt_c_bconv_create(const uint64_t* q_from, uint32_t L, const uint64_t* p_to, uint32_t K,
+                              ntt_c_bconv_plan** out);
+/* d_out (K x n) from d_in (L x n, canonical residues); no overlap. */
+ntt_status ntt_c_bconv(const ntt_c_bconv_plan* plan, uint32_t* d_out, const uint32_t* d_in, uint64_t n, void* stream);
+ntt_status ntt_c_bconv_destroy(ntt_c_bconv_plan* plan);
+
+/* Per-limb multiply-accumulate of an inner product over dnum <= 32 pairs:
+ * out[l][c] = (accumulate ? out[l][c] : 0) + sum_k a[k][l][c] * b[k][l][c] mod q_l,
+ * a and b laid out [k][l][c] (dnum x L x n), inputs canonical; out may not
+ * overlap a or b. */
+ntt_status ntt_c_pmac_create(const uint64_t* q, uint32_t L, ntt_c_pmac_plan** out);
+ntt_status ntt_c_pmac(const ntt_c_pmac_plan* plan, uint32_t* d_out, const uint32_t* d_a, const uint32_t* d_b,
+                      uint32_t dnum, uint64_t n, int accumulate, void* stream);
+ntt_status ntt_c_pmac_destroy(ntt_c_pmac_plan* plan);
+
 /* ---------------- device-memory and timing helpers for hosts without a HIP
  * binding (tests, benchmarks, FFI callers) ---------------- */
 ntt_status ntt_c_device_count(int* count);

@@ -1438,6 +1438,7 @@ extern "C" ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc
 #include "ntt_serve.inc"
 #include "ntt_tune.inc"
 #include "ntt_arb.inc"
+#include "ntt_rns.inc"
 
 // ------------------------------------------------------------------ device helpers
 extern "C" ntt_status ntt_c_device_count(int* c) {

@@ -107,6 +107,12 @@ def lib():
         "ntt_c_event_destroy": [vp],
         "ntt_c_event_record": [vp, vp],
         "ntt_c_event_elapsed_ms": [P(ctypes.c_float), vp, vp],
+        "ntt_c_bconv_create": [P(u64), u32, P(u64), u32, P(vp)],
+        "ntt_c_bconv": [vp, vp, vp, u64, vp],
+        "ntt_c_bconv_destroy": [vp],
+        "ntt_c_pmac_create": [P(u64), u32, P(vp)],
+        "ntt_c_pmac": [vp, vp, vp, vp, u32, u64, i32, vp],
+        "ntt_c_pmac_destroy": [vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -319,6 +325,49 @@ def rowmaj_from_pretransposed(dst, src, n1, n2, stream=None):
 
 
 # ---------------------------------------------------------------- descriptor API
+def _u64_array(xs):
+    xs = [int(v) for v in xs]
+    return (ctypes.c_uint64 * len(xs))(*xs), len(xs)
+
+
+class Bconv:
+    """Fast RNS base conversion plan (include/ntt_c.h, ntt_c_bconv_*): limb-major
+    uint32 residues, L source moduli to K target moduli."""
+
+    def __init__(self, q_from, p_to):
+        qa, self.L = _u64_array(q_from)
+        pa, self.K = _u64_array(p_to)
+        p = ctypes.c_void_p()
+        check(lib().ntt_c_bconv_create(qa, self.L, pa, self.K, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def run(self, out, inp, n, stream=None):
+        check(lib().ntt_c_bconv(self.ptr, out.ptr, inp.ptr, n, _sp(stream)))
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().ntt_c_bconv_destroy(self.ptr))
+            self.ptr = None
+
+
+class Pmac:
+    """Per-limb multiply-accumulate plan (ntt_c_pmac_*): out[l] (+)= sum_k a[k][l] * b[k][l] mod q_l."""
+
+    def __init__(self, q):
+        qa, self.L = _u64_array(q)
+        p = ctypes.c_void_p()
+        check(lib().ntt_c_pmac_create(qa, self.L, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def run(self, out, a, b, dnum, n, accumulate=False, stream=None):
+        check(lib().ntt_c_pmac(self.ptr, out.ptr, a.ptr, b.ptr, dnum, n, int(bool(accumulate)), _sp(stream)))
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().ntt_c_pmac_destroy(self.ptr))
+            self.ptr = None
+
+
 class Context:
     def __init__(self):
         p = ctypes.c_void_p()

@@ -18,6 +18,12 @@
  *                     and the direct sum  A_k = sum_j a_j psi^{j(2k+1)}.
  *   orc_arb_*       : K-limb Montgomery identities, Goldilocks/BN254 root orders,
  *                     and agreement with the 64-bit path on the same prime.
+ *   orc_bconv, orc_pmac : no reference code to restate (the reference names
+ *                     BConv/IP and PMAC at src/ntt_optimized.cu:510-513 as
+ *                     absent), so parity with the reference is unpinned; they
+ *                     restate the standard definitions and are pinned by big-
+ *                     integer CRT (S = sum t_i Q/q_i is X + uQ with 0 <= u < L)
+ *                     and by direct Python-integer sums in tests/test_oracle.py.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -361,4 +367,57 @@ void orc_arb_ntt_mont(uint32_t* a, const uint32_t* tw_m, uint32_t n, const uint3
                 arb_sub(a + (size_t)(g + k + half) * K, u, t, p, K);
             }
     }
+}
+
+/* Fast RNS base conversion from base {q_i} (i < L) to base {p_j} (j < K), the
+ * BConv of the COOP step named at reference src/ntt_optimized.cu:510-513
+ * (no code there): with Q = prod q_i and qh_i = Q / q_i,
+ *   t_i = x_i * (qh_i^-1 mod q_i) mod q_i,   y_j = sum_i t_i * (qh_i mod p_j) mod p_j,
+ * so y_j = (X + u*Q) mod p_j for the CRT value X of x and some 0 <= u < L.
+ * Limb-major layout: in[i*n + c], out[j*n + c]; moduli are primes. */
+void orc_bconv(const uint32_t* in, uint32_t* out, uint64_t n, const uint64_t* q, uint32_t L, const uint64_t* p,
+               uint32_t K) {
+    uint64_t* w = (uint64_t*)malloc(sizeof(uint64_t) * (L ? L : 1));
+    uint64_t* m = (uint64_t*)malloc(sizeof(uint64_t) * (L && K ? (uint64_t)L * K : 1));
+    for (uint32_t i = 0; i < L; i++) {
+        uint64_t h = 1;
+        for (uint32_t k = 0; k < L; k++)
+            if (k != i) h = (uint64_t)((u128)h * (q[k] % q[i]) % q[i]);
+        w[i] = orc_mod_inverse(h, q[i]);
+        for (uint32_t j = 0; j < K; j++) {
+            uint64_t g = 1;
+            for (uint32_t k = 0; k < L; k++)
+                if (k != i) g = (uint64_t)((u128)g * (q[k] % p[j]) % p[j]);
+            m[(uint64_t)j * L + i] = g;
+        }
+    }
+    uint64_t* t = (uint64_t*)malloc(sizeof(uint64_t) * (L ? L : 1));
+    for (uint64_t c = 0; c < n; c++) {
+        for (uint32_t i = 0; i < L; i++) t[i] = (uint64_t)((u128)in[(uint64_t)i * n + c] * w[i] % q[i]);
+        for (uint32_t j = 0; j < K; j++) {
+            u128 acc = 0;
+            for (uint32_t i = 0; i < L; i++) acc += (u128)t[i] * m[(uint64_t)j * L + i];
+            out[(uint64_t)j * n + c] = (uint32_t)(acc % p[j]);
+        }
+    }
+    free(t);
+    free(m);
+    free(w);
+}
+
+/* Multiply-accumulate over dnum polynomial pairs per RNS limb, the inner
+ * product (IP) / PMAC step named at reference src/ntt_optimized.cu:510-513
+ * (no code there):  out[l][c] = (acc ? out[l][c] : 0) + sum_k a[k][l][c] b[k][l][c]  mod q_l,
+ * layouts a[(k*L + l)*n + c], out[l*n + c]; inputs canonical. */
+void orc_pmac(uint32_t* out, const uint32_t* a, const uint32_t* b, uint32_t dnum, uint32_t L, uint64_t n,
+              const uint64_t* q, int acc) {
+    for (uint32_t l = 0; l < L; l++)
+        for (uint64_t c = 0; c < n; c++) {
+            u128 s = acc ? out[(uint64_t)l * n + c] : 0;
+            for (uint32_t k = 0; k < dnum; k++) {
+                const uint64_t at = ((uint64_t)k * L + l) * n + c;
+                s += (u128)a[at] * b[at];
+            }
+            out[(uint64_t)l * n + c] = (uint32_t)(s % q[l]);
+        }
 }

@@ -52,6 +52,8 @@ def lib():
         L.orc_ntt_negacyclic.argtypes = [U64P, u64, u64, ctypes.c_int]
         L.orc_pointwise_mul.argtypes = [U64P, U64P, U64P, u64, u64]
         L.orc_negacyclic_schoolbook.argtypes = [U64P, U64P, U64P, u64, u64]
+        L.orc_bconv.argtypes = [U32P, U32P, u64, U64P, ctypes.c_uint32, U64P, ctypes.c_uint32]
+        L.orc_pmac.argtypes = [U32P, U32P, U32P, ctypes.c_uint32, ctypes.c_uint32, u64, U64P, ctypes.c_int]
         L.orc_dft_naive.argtypes = [U64P, U64P, u64, u64, u64]
         L.orc_ntt_pow_twiddle.argtypes = [U64P, u64, u64, u64]
         L.orc_random_poly.argtypes = [U64P, u64, u64, u64]
@@ -144,6 +146,30 @@ def negacyclic_schoolbook(a, b, q):
     b = np.ascontiguousarray(b, dtype=np.uint64)
     r = np.empty_like(a)
     lib().orc_negacyclic_schoolbook(_p64(a), _p64(b), _p64(r), a.size, q)
+    return r
+
+
+def bconv(x, q_from, p_to):
+    """Fast base conversion of limb-major x (L x n uint32) to the K moduli p_to."""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    L, n = x.shape
+    q = np.ascontiguousarray(q_from, dtype=np.uint64)
+    p = np.ascontiguousarray(p_to, dtype=np.uint64)
+    y = np.empty((p.size, n), dtype=np.uint32)
+    lib().orc_bconv(x.ctypes.data_as(U32P), y.ctypes.data_as(U32P), n, _p64(q), L, _p64(p), p.size)
+    return y
+
+
+def pmac(a, b, q, out=None):
+    """(out +) sum_k a[k] * b[k] mod q per limb; a, b: dnum x L x n uint32."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    dnum, L, n = a.shape
+    qq = np.ascontiguousarray(q, dtype=np.uint64)
+    acc = out is not None
+    r = np.ascontiguousarray(out, dtype=np.uint32).copy() if acc else np.empty((L, n), dtype=np.uint32)
+    lib().orc_pmac(r.ctypes.data_as(U32P), a.ctypes.data_as(U32P), b.ctypes.data_as(U32P), dnum, L, n, _p64(qq),
+                   int(acc))
     return r
 
 

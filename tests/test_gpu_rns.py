@@ -1,0 +1,67 @@
+"""RNS kernels around the transform: fast base conversion and the per-limb
+multiply-accumulate, bit-exact against the oracle (oracle/ntt_oracle.c
+orc_bconv / orc_pmac, themselves pinned by CRT in tests/test_oracle.py).
+The reference has no code for either (src/ntt_optimized.cu:510-513)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+U32 = np.uint32
+
+
+def _primes(orc, count, below=1 << 31):
+    return [int(v) for v in orc.ntt_primes(below, 17, count)]
+
+
+def _residues(rng, mods, n):
+    return np.stack([rng.integers(0, m, size=n, dtype=np.uint64) for m in mods]).astype(U32)
+
+
+@pytest.mark.parametrize("L,K", [(1, 1), (3, 2), (8, 8), (24, 24), (32, 64), (5, 40)])
+@pytest.mark.parametrize("n", [1, 257, 1 << 16])
+def test_bconv_vs_oracle(nt, orc, L, K, n):
+    primes = _primes(orc, L + K)
+    q, p = primes[:L], primes[L:]
+    x = _residues(np.random.default_rng(L * 1000 + K + n), q, n)
+    plan = nt.Bconv(q, p)
+    din = nt.DeviceBuffer.from_array(x)
+    dout = nt.DeviceBuffer(K * n * 4)
+    plan.run(dout, din, n)
+    nt.device_sync()
+    assert np.array_equal(dout.download(U32, K * n).reshape(K, n), orc.bconv(x, q, p))
+    plan.destroy()
+    din.free()
+    dout.free()
+
+
+@pytest.mark.parametrize("dnum,L", [(1, 1), (3, 4), (8, 24), (32, 2)])
+@pytest.mark.parametrize("acc", [False, True])
+def test_pmac_vs_oracle(nt, orc, dnum, L, acc):
+    n = 3000
+    q = _primes(orc, L)
+    rng = np.random.default_rng(dnum * 100 + L)
+    a = np.stack([_residues(rng, q, n) for _ in range(dnum)])
+    b = np.stack([_residues(rng, q, n) for _ in range(dnum)])
+    base = _residues(rng, q, n)
+    plan = nt.Pmac(q)
+    da, db = nt.DeviceBuffer.from_array(a), nt.DeviceBuffer.from_array(b)
+    dout = nt.DeviceBuffer.from_array(base)
+    plan.run(dout, da, db, dnum, n, accumulate=acc)
+    nt.device_sync()
+    assert np.array_equal(dout.download(U32, L * n).reshape(L, n), orc.pmac(a, b, q, base if acc else None))
+    plan.destroy()
+    for x in (da, db, dout):
+        x.free()
+
+
+def test_rns_plans_reject_bad_moduli(nt, orc):
+    q = _primes(orc, 3)
+    with pytest.raises(nt.NttError):
+        nt.Bconv([q[0], q[0]], q[1:])               # repeated source modulus
+    with pytest.raises(nt.NttError):
+        nt.Bconv(q, [q[0] + 2])                     # not a prime (even)
+    with pytest.raises(nt.NttError):
+        nt.Bconv(_primes(orc, 33), q)               # more than 32 source moduli
+    with pytest.raises(nt.NttError):
+        nt.Pmac([(1 << 31) + 11])                   # above 2^31

@@ -200,3 +200,60 @@ def test_prime_sets(orc):
         assert sympy.isprime(p)
     for c in (2147352577 + 2, 561, 1105, 3215031751):
         assert orc.is_prime(c) == sympy.isprime(c)
+
+
+# ---- fast base conversion and PMAC (no reference code: pinned by CRT and
+# direct Python-integer sums, independent of the oracle's modular loops)
+
+def _rns_primes(orc, count, below=1 << 31, two_adic=17):
+    return [int(v) for v in orc.ntt_primes(below, two_adic, count)]
+
+
+@pytest.mark.parametrize("L,K", [(1, 1), (3, 2), (5, 4), (12, 9)])
+def test_bconv_crt_identity(orc, L, K):
+    primes = _rns_primes(orc, L + K)
+    q, p = primes[:L], primes[L:]
+    n = 40
+    rng = np.random.default_rng(L * 100 + K)
+    x = np.stack([rng.integers(0, qi, size=n, dtype=np.uint64) for qi in q]).astype(np.uint32)
+    y = orc.bconv(x, q, p)
+    Q = 1
+    for qi in q:
+        Q *= qi
+    for c in range(n):
+        xs = [int(x[i, c]) for i in range(L)]
+        X = sum(xs[i] * (Q // q[i]) * pow(Q // q[i], -1, q[i]) for i in range(L)) % Q  # CRT
+        t = [xs[i] * pow((Q // q[i]) % q[i], -1, q[i]) % q[i] for i in range(L)]
+        S = sum(t[i] * (Q // q[i]) for i in range(L))
+        assert S % Q == X and 0 <= (S - X) // Q < L
+        assert [int(y[j, c]) for j in range(K)] == [S % pj for pj in p]
+
+
+def test_bconv_small_values_are_exact_up_to_uq(orc):
+    # every limb holds the small integer X: the output is X + u Q mod p_j for
+    # one u in [0, L) shared by all j (u = 0 for X = 0)
+    q = _rns_primes(orc, 4)
+    p = _rns_primes(orc, 3, below=q[-1])
+    Q = 1
+    for qi in q:
+        Q *= qi
+    for X in (0, 1, 12345):
+        x = np.array([[X % qi] for qi in q], dtype=np.uint32)
+        y = [int(v) for v in orc.bconv(x, q, p)[:, 0]]
+        us = [u for u in range(len(q)) if y == [(X + u * Q) % pj for pj in p]]
+        assert len(us) == 1 and (X != 0 or us == [0])
+
+
+@pytest.mark.parametrize("dnum,L,acc", [(1, 1, False), (3, 4, False), (5, 2, True)])
+def test_pmac_direct_sum(orc, dnum, L, acc):
+    q = _rns_primes(orc, L)
+    n = 33
+    rng = np.random.default_rng(dnum * 10 + L)
+    a = np.stack([np.stack([rng.integers(0, qi, size=n, dtype=np.uint64) for qi in q]) for _ in range(dnum)]).astype(np.uint32)
+    b = np.stack([np.stack([rng.integers(0, qi, size=n, dtype=np.uint64) for qi in q]) for _ in range(dnum)]).astype(np.uint32)
+    base = np.stack([rng.integers(0, qi, size=n, dtype=np.uint64) for qi in q]).astype(np.uint32) if acc else None
+    r = orc.pmac(a, b, q, base)
+    for l in range(L):
+        for c in range(n):
+            s = (int(base[l, c]) if acc else 0) + sum(int(a[k, l, c]) * int(b[k, l, c]) for k in range(dnum))
+            assert int(r[l, c]) == s % q[l]
