@@ -204,6 +204,19 @@ __device__ __forceinline__ void lds_st(uint32_t off, T v) {
     *reinterpret_cast<lds_p<T>>((size_t)off) = v;
 }
 
+// Load through a global-address-space pointer: the job's table pointers are
+// generic (void*), and a generic load is a flat load, which the compiler
+// must also order against LDS accesses.
+// (The host pass of the compiler has no global address space.)
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(1))) T*)p;
+#else
+    return *p;
+#endif
+}
+
 // A table of Shoup pairs staged in LDS at byte offset `off` (a multiple of the
 // pair's size): indexes and offsets like the global pointer it replaces.  A
 // pair moves as one aligned 8- or 16-byte access.
@@ -628,8 +641,10 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
 // Tile load into LDS (row-major, swizzled): PER elements per thread and
 // sweep, PRED per-element bounds predicates, LR/LC as in store_loop; MUL
 // stores in[i]*in2[i] mod q (JF_MUL_IN).
-template <int PER, bool PRED, int LR, int LC, class Sin, class V, bool MUL = false>
-__device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint32_t col0) {
+//   pre() runs once, after the first batch's global loads are issued and before
+//   their LDS stores, so that loads it issues share their latency.
+template <int PER, bool PRED, int LR, int LC, class Sin, class V, bool MUL = false, class Pre>
+__device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint32_t col0, Pre pre) {
     constexpr uint32_t VB = sizeof(V);
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
     const uint32_t R = 1u << log_r, C = 1u << log_c;
@@ -684,6 +699,7 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
             fetch(in + off, ok, v[u]);
             if (MUL) fetch(in2 + off, ok, w[MUL ? u : 0]);
         }
+        if (bb == 0) pre();
 #pragma unroll
         for (int u = 0; u < UN; u++) {
             const uint32_t ui = bb + (uint32_t)u * sweep;
@@ -740,20 +756,20 @@ __device__ __forceinline__ void stage_tables(const PassJob& J, const Staged<type
         const TwP<V>* zsrc = mf ? reinterpret_cast<const TwP<V>*>(reinterpret_cast<const char*>(J.mz) + kMfTwOff)
                                 : reinterpret_cast<const TwP<V>*>(J.z);
         const uint32_t nz = mf ? mf_tw_entries(log_r) : R;
-        for (uint32_t i = tid; i < nz; i += T) t.zl.set(i, zsrc[i]);
+        for (uint32_t i = tid; i < nz; i += T) t.zl.set(i, gld(zsrc + i));
     }
     if (P4 && (flags & JF_POST_ROW)) {
         const TwP<V>* gr = reinterpret_cast<const TwP<V>*>(J.tw_row) + ((size_t)(col0 >> J.row_shift) << log_r);
-        for (uint32_t i = tid; i < R; i += T) t.pl.set(i, gr[i]);
+        for (uint32_t i = tid; i < R; i += T) t.pl.set(i, gld(gr + i));
     } else if (P4 && !row_only && (flags & JF_POST_LDS)) {
         const uint32_t n_lo = 1u << J.lo_bits, n_mid = J.mid_bits ? 1u << J.mid_bits : 0u;
         const uint32_t n_hi = (J.e_mask >> (J.lo_bits + J.mid_bits)) + 1u;
         const TwP<V>* gl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
         const TwP<V>* gm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
         const TwP<V>* gh = reinterpret_cast<const TwP<V>*>(J.tw_hi);
-        for (uint32_t i = tid; i < n_lo; i += T) t.pl.set(tsw(i), gl[i]);
-        for (uint32_t i = tid; i < n_mid; i += T) t.pm.set(tsw(i), gm[i]);
-        for (uint32_t i = tid; i < n_hi; i += T) t.ph.set(tsw(i), gh[i]);
+        for (uint32_t i = tid; i < n_lo; i += T) t.pl.set(tsw(i), gld(gl + i));
+        for (uint32_t i = tid; i < n_mid; i += T) t.pm.set(tsw(i), gld(gm + i));
+        for (uint32_t i = tid; i < n_hi; i += T) t.ph.set(tsw(i), gld(gh + i));
     }
 }
 
@@ -761,7 +777,7 @@ __device__ __forceinline__ void stage_tables(const PassJob& J, const Staged<type
 // rounds on the matrix cores), then the store with the pass's post-multiply.
 template <class M, class Sout, int MAXM, bool P4, int LR, int LC, bool MF>
 __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<typename M::V>& t, uint32_t poly,
-                                             uint32_t col0, bool full) {
+                                             uint32_t col0, bool full, const MfMat& mat) {
     using V = typename M::V;
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
     const uint32_t total = 1u << (log_r + log_c);
@@ -773,8 +789,6 @@ __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<type
     if constexpr (MF && std::is_same<M, Mod32>::value) {
         const TwP<V>* twg = reinterpret_cast<const TwP<V>*>(reinterpret_cast<const char*>(J.mz) + kMfTwOff);
         const bool canon = J.post == POST_NONE;
-        MfMat mat;
-        mf_load_mat(J.mz, mat);
         if (flags & JF_Z_LDS) mf_levels(mod, zg, mat, t.zl, log_r, log_c, cyc, canon);
         else mf_levels(mod, zg, mat, twg, log_r, log_c, cyc, canon);
     } else if (LR) {
@@ -812,25 +826,33 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
     const bool full = col0 + C <= J.ncols;  // every column of the tile exists
     const uint32_t flags = J.flags;
     const Staged<V> t = staged_of<M, P4, LR, MF>(J);
-    stage_tables<M, P4, LR, MF>(J, t, col0, false);
+    // The job's tables and matrix are fetched while the tile's first global
+    // loads are in flight (load_loop's pre), so a tile waits for one memory
+    // round trip before its LDS writes instead of two, and the matrix-core
+    // rounds none after the barrier.
+    MfMat mat{};
+    auto pre = [&] {
+        stage_tables<M, P4, LR, MF>(J, t, col0, false);
+        if constexpr (MF && std::is_same<M, Mod32>::value) mf_load_mat(J.mz, mat);
+    };
 
     // ---- load ----
     // Predicates are needed for partial tiles and for tiles smaller than one sweep.
     const bool pred_l = !full || total < ((flags & JF_VEC4) ? 4u : 1u) * kNT;
     if (LR) {  // shape-specialised launches are full tiles with vector loads
-        load_loop<4, false, LR, LC, Sin, V>(J, poly, col0);
+        load_loop<4, false, LR, LC, Sin, V>(J, poly, col0, pre);
     } else if (flags & JF_MUL_IN) {  // pointwise product fused into the load (predicated: rare path)
-        if (flags & JF_VEC4) load_loop<4, true, LR, LC, Sin, V, true>(J, poly, col0);
-        else load_loop<1, true, LR, LC, Sin, V, true>(J, poly, col0);
+        if (flags & JF_VEC4) load_loop<4, true, LR, LC, Sin, V, true>(J, poly, col0, pre);
+        else load_loop<1, true, LR, LC, Sin, V, true>(J, poly, col0, pre);
     } else if (flags & JF_VEC4) {
-        if (pred_l) load_loop<4, true, LR, LC, Sin, V>(J, poly, col0);
-        else load_loop<4, false, LR, LC, Sin, V>(J, poly, col0);
+        if (pred_l) load_loop<4, true, LR, LC, Sin, V>(J, poly, col0, pre);
+        else load_loop<4, false, LR, LC, Sin, V>(J, poly, col0, pre);
     } else {
-        if (pred_l) load_loop<1, true, LR, LC, Sin, V>(J, poly, col0);
-        else load_loop<1, false, LR, LC, Sin, V>(J, poly, col0);
+        if (pred_l) load_loop<1, true, LR, LC, Sin, V>(J, poly, col0, pre);
+        else load_loop<1, false, LR, LC, Sin, V>(J, poly, col0, pre);
     }
     lds_barrier();
-    ct_and_store<M, Sout, MAXM, P4, LR, LC, MF>(J, t, poly, col0, full);
+    ct_and_store<M, Sout, MAXM, P4, LR, LC, MF>(J, t, poly, col0, full, mat);
 }
 
 // Workgroup index -> logical tile so that consecutive tiles run on the same

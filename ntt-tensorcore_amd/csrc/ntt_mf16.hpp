@@ -54,12 +54,12 @@ __device__ __forceinline__ void mf_load_mat(const void* blk, MfMat& M) {
     const uint32_t lane = threadIdx.x & 63u;
     const mf_v4i* f = reinterpret_cast<const mf_v4i*>(blk);
 #pragma unroll
-    for (int c = 0; c < 4; c++) M.a[c] = f[c * 64 + lane];
-    M.h = f[256 + lane];
+    for (int c = 0; c < 4; c++) M.a[c] = gld(f + c * 64 + lane);
+    M.h = gld(f + 256 + lane);
     const uint32_t* hd = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(blk) + kMfHdrOff);
-    M.l0 = hd[0];
-    M.cm = hd[1];
-    M.cmq = hd[2];
+    M.l0 = gld(hd);
+    M.cm = gld(hd + 1);
+    M.cmq = gld(hd + 2);
 }
 
 // Entries of the round twiddle tables of an R = 2^log_r pass (first MFMA
