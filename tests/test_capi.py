@@ -91,6 +91,28 @@ def test_null_arguments_rejected(nt):
     assert L.ntt_c_rowmaj_to_pretransposed(ctypes.c_void_p(16), ctypes.c_void_p(16), 64, 64, None) == 1
 
 
+def test_converter_and_rns_arguments_checked_on_the_host(nt):
+    """Invalid calls of the width converters and the RNS plans return
+    NTT_ERR_INVALID_ARG (1) or NTT_ERR_UNSUPPORTED before any device work."""
+    L = nt.lib()
+    p16 = ctypes.c_void_p(256)
+    assert L.ntt_c_u64_to_u32(None, p16, 8, 2013265921, None) == 1
+    assert L.ntt_c_u64_to_u32(p16, ctypes.c_void_p(4096), 8, 1 << 32, None) == 1     # q too large
+    assert L.ntt_c_u64_to_u32(ctypes.c_void_p(260), ctypes.c_void_p(4096), 8, 2013265921, None) == 1  # misaligned
+    assert L.ntt_c_u32_to_u64(ctypes.c_void_p(4096), ctypes.c_void_p(4096 + 16), 64, None) == 1    # overlap
+    q = (ctypes.c_uint64 * 3)(2013265921, 2013265921, 15)
+    out = ctypes.c_void_p()
+    assert L.ntt_c_bconv_create(q, 2, q, 1, ctypes.byref(out)) == 1          # repeated source modulus
+    assert L.ntt_c_bconv_create(q, 1, ctypes.cast(ctypes.byref(q, 16), ctypes.POINTER(ctypes.c_uint64)), 1,
+                                ctypes.byref(out)) == 1                       # 15 is not prime
+    assert L.ntt_c_bconv_create(q, 0, q, 1, ctypes.byref(out)) != 0          # no source moduli
+    assert L.ntt_c_pmac_create(ctypes.cast(ctypes.byref(q, 16), ctypes.POINTER(ctypes.c_uint64)), 1,
+                               ctypes.byref(out)) == 1
+    assert L.ntt_c_bconv(None, p16, p16, 8, None) == 1
+    assert L.ntt_c_pmac(None, p16, p16, p16, 1, 8, 0, None) == 1
+    assert out.value is None
+
+
 def test_every_declared_function_has_a_prototype(nt):
     """A function without argtypes would get 64-bit pointers as C ints."""
     import ntt_arb  # noqa: F401  (registers the K-limb prototypes)
