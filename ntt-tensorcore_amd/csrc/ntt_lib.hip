@@ -556,12 +556,12 @@ static uint64_t plan_scratch_elems(const Plan& P, Store st) {
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
-// Load/store order, vector width and LDS staging flags of a job.
 // Entries of the post-multiply tables (lo, mid, hi) of a POST_FOURSTEP job.
 static uint32_t post_table_entries(const PassJob& J) {
     return (1u << J.lo_bits) + (J.mid_bits ? 1u << J.mid_bits : 0u) + (J.e_mask >> (J.lo_bits + J.mid_bits)) + 1u;
 }
 
+// Load/store order, vector width and LDS staging flags of a job.
 static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma) {
     const uint32_t R = 1u << J.log_r, C = 1u << J.log_c;
     uint32_t f = mfma ? (uint32_t)JF_MFMA : 0u;
@@ -816,8 +816,9 @@ static uint32_t group_shape(const PassJob* jobs, size_t n, Arith ar, Store sin, 
     return sh;
 }
 
-// Runs plan P on npoly polynomials at data (contiguous, stride n); with mul_a,
-// on mul_a*mul_b mod q elementwise, the result going to data (make_jobs).
+// Runs plan P on npoly polynomials at data (stride n, or dstride); with
+// mul_a, on mul_a*mul_b mod q elementwise, the result going to data
+// (make_jobs).
 static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, hipStream_t stream,
                            const void* mul_a = nullptr, const void* mul_b = nullptr, int64_t dstride = 0) {
     PassJob jobs[4];
@@ -846,7 +847,7 @@ struct ConfigImpl {
     }
 };
 
-// Workspace for two-pass transforms, one buffer per stream so that one config
+// Workspace for multi-pass transforms, one buffer per stream so that one config
 // can serve several streams at once; grown on demand, never shrunk.
 static ntt_status scratch_for(ConfigImpl* I, hipStream_t stream, uint64_t bytes, void** out) {
     *out = nullptr;
