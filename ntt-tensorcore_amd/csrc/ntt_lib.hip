@@ -494,7 +494,8 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
         s.log_c = std::min(sp.log_e > lr ? sp.log_e - lr : 0u, log_n - lr);
         const uint64_t fac = (i == 0 && nega && !inv) ? powmod(p, N >> lr, q) : 1;
         s.cyc = fac == 1;
-        // MFMA passes: R = 4096 any width; R = 64 needs >= 16 columns; never single-pass
+        // Matrix-core rounds where the split asks for them and mfma_ok allows
+        // (make_jobs also needs a 16-unit group, >= 2^8 elements, per tile).
         const std::vector<uint64_t> Z = ct_table(lr, powmod(w, N >> lr, q), fac, q);
         s.z_off = tb.add(Z);
         s.log_c = std::min(s.log_c, max_log_e(arith) > lr ? max_log_e(arith) - lr : 0u);  // tile fits LDS
