@@ -66,6 +66,24 @@ hipError_t launch_pass_kernel(const PassJob* djobs, uint32_t njobs, const PassJo
 #define NTT_SHAPE_INSTANCE_EXTERN(LR, LC) NTT_SHAPE_INSTANCE_ALL(extern, LR, LC)
 #define NTT_SHAPE_INSTANCE(LR, LC) NTT_SHAPE_INSTANCE_ALL(, LR, LC)
 
+// Shapes with matrix-core-round instances for mixed storage: the first pass
+// of a transform on uint64 data (uint64 in, 32-bit out, with post) and its
+// last pass (32-bit in, uint64 out, with or without post).  These are the
+// shapes the tuned plans of single transforms use; other shapes run the
+// generic kernel.
+#define NTT_SHAPES_U64_A(X) X(8, 5) X(8, 4) X(7, 5) X(9, 4) X(7, 6) X(10, 4)
+#define NTT_SHAPES_U64_B(X) X(10, 2) X(10, 3) X(9, 3) X(11, 2) X(11, 3) X(12, 2)
+#define NTT_SHAPES_U64(X) NTT_SHAPES_U64_A(X) NTT_SHAPES_U64_B(X)
+#define NTT_SHAPE_U64_INSTANCE_ALL(EXT, LR, LC)                                                                 \
+    EXT template hipError_t launch_pass_kernel<Mod32, uint64_t, uint32_t, true, LR, LC, true>(                  \
+        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);                             \
+    EXT template hipError_t launch_pass_kernel<Mod32, uint32_t, uint64_t, true, LR, LC, true>(                  \
+        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);                             \
+    EXT template hipError_t launch_pass_kernel<Mod32, uint32_t, uint64_t, false, LR, LC, true>(                 \
+        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);
+#define NTT_SHAPE_U64_INSTANCE_EXTERN(LR, LC) NTT_SHAPE_U64_INSTANCE_ALL(extern, LR, LC)
+#define NTT_SHAPE_U64_INSTANCE(LR, LC) NTT_SHAPE_U64_INSTANCE_ALL(, LR, LC)
+
 // Instantiated in pass_*.hip.
 NTT_PASS_INSTANCES_MF(extern, uint32_t, uint32_t)
 NTT_PASS_INSTANCES_MF(extern, uint32_t, uint64_t)
@@ -73,5 +91,6 @@ NTT_PASS_INSTANCES_MF(extern, uint64_t, uint32_t)
 NTT_PASS_INSTANCES_MF(extern, uint64_t, uint64_t)
 NTT_PASS_INSTANCES(extern, Mod64, uint64_t, uint64_t)
 NTT_SHAPES(NTT_SHAPE_INSTANCE_EXTERN)
+NTT_SHAPES_U64(NTT_SHAPE_U64_INSTANCE_EXTERN)
 
 }  // namespace nttk

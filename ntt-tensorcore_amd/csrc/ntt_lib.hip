@@ -182,11 +182,23 @@ static Split heuristic_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight)
     return sp;
 }
 
+// The tuned entry for (arith, log_n, elements in flight), else the tuned
+// entry for the same transform at the nearest number of elements in flight
+// (the split that won at another batch size), else the heuristic.
 static Split choose_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight) {
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tuned.find(std::make_tuple((int)ar, log_n, ilog2_floor(std::max<uint64_t>(elems_in_flight, 1))));
+        const uint32_t le = ilog2_floor(std::max<uint64_t>(elems_in_flight, 1));
+        auto it = g_tuned.find(std::make_tuple((int)ar, log_n, le));
         if (it != g_tuned.end()) return it->second;
+        const Split* best = nullptr;
+        uint32_t best_d = ~0u;
+        for (const auto& kv : g_tuned) {
+            if (std::get<0>(kv.first) != (int)ar || std::get<1>(kv.first) != log_n) continue;
+            const uint32_t e = std::get<2>(kv.first), d = e > le ? e - le : le - e;
+            if (d < best_d) best = &kv.second, best_d = d;
+        }
+        if (best) return *best;
     }
     return heuristic_split(log_n, ar, elems_in_flight);
 }
@@ -752,6 +764,29 @@ static ntt_status launch_generic(Arith ar, Store sin, Store sout, const PassJob*
     return NTT_OK;
 }
 
+// Shape-specialised launch for uint64 data (first pass uint64 -> 32-bit with
+// post, last pass 32-bit -> uint64), matrix-core rounds; false if none.
+template <bool P4>
+static bool launch_shape_u64(Store sin, uint32_t shape, const PassJob* djobs, uint32_t njobs, const PassJob& inl,
+                             uint32_t nblocks, uint32_t lds, hipStream_t stream, hipError_t* e) {
+    switch (shape) {
+#define NTT_SHAPE_CASE(LR, LC)                                                                                    \
+    case (LR << 8) | LC:                                                                                           \
+        if (sin == Store::U64) {                                                                                   \
+            if (!P4) return false;                                                                                 \
+            *e = launch_pass_kernel<Mod32, uint64_t, uint32_t, true, LR, LC, true>(djobs, njobs, inl, nblocks, lds, \
+                                                                                    stream);                       \
+        } else {                                                                                                   \
+            *e = launch_pass_kernel<Mod32, uint32_t, uint64_t, P4, LR, LC, true>(djobs, njobs, inl, nblocks, lds,   \
+                                                                                  stream);                         \
+        }                                                                                                          \
+        return true;
+        NTT_SHAPES_U64(NTT_SHAPE_CASE)
+#undef NTT_SHAPE_CASE
+        default: return false;
+    }
+}
+
 // Shape-specialised launch; false if the shape has no instance.
 template <bool P4, bool MF>
 static bool launch_shape(uint32_t shape, const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t nblocks,
@@ -776,10 +811,15 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, co
     const bool p4 = kind & JF_POST4, mf = kind & JF_MFMA;
     if (shape) {
         hipError_t e = hipSuccess;
-        const bool found = p4 ? (mf ? launch_shape<true, true>(shape, djobs, njobs, inl, nblocks, lds, stream, &e)
-                                    : launch_shape<true, false>(shape, djobs, njobs, inl, nblocks, lds, stream, &e))
-                              : (mf ? launch_shape<false, true>(shape, djobs, njobs, inl, nblocks, lds, stream, &e)
-                                    : launch_shape<false, false>(shape, djobs, njobs, inl, nblocks, lds, stream, &e));
+        bool found;
+        if (sin != Store::U32 || sout != Store::U32)
+            found = mf && (p4 ? launch_shape_u64<true>(sin, shape, djobs, njobs, inl, nblocks, lds, stream, &e)
+                              : launch_shape_u64<false>(sin, shape, djobs, njobs, inl, nblocks, lds, stream, &e));
+        else
+            found = p4 ? (mf ? launch_shape<true, true>(shape, djobs, njobs, inl, nblocks, lds, stream, &e)
+                             : launch_shape<true, false>(shape, djobs, njobs, inl, nblocks, lds, stream, &e))
+                       : (mf ? launch_shape<false, true>(shape, djobs, njobs, inl, nblocks, lds, stream, &e)
+                             : launch_shape<false, false>(shape, djobs, njobs, inl, nblocks, lds, stream, &e));
         if (!found) return fail(NTT_ERR_DEVICE, "internal: no kernel for this tile shape");
         HIP_TRY(e);
         return NTT_OK;
@@ -790,19 +830,34 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, co
               : launch_generic<false, false>(ar, sin, sout, djobs, njobs, inl, nblocks, lds, stream);
 }
 
-// Shape-specialised kernel for job J, as (log_r << 8) | log_c, or 0: 32-bit
-// coefficients in and out, vector loads and stores, full tiles, post tables
-// in LDS, and a shape that has an instance.
+// Shape-specialised kernel for job J, as (log_r << 8) | log_c, or 0: vector
+// loads and stores, full tiles, post tables in LDS, no fused product, and a
+// shape that has an instance for the job's storage (32-bit in and out, or a
+// matrix-core first or last pass on uint64 data).
 static uint32_t pass_shape_of(const PassJob& J, Arith ar, Store sin, Store sout) {
-    if (ar != Arith::M32 || sin != Store::U32 || sout != Store::U32) return 0;
+    if (ar != Arith::M32) return 0;
     if (J.flags & JF_MUL_IN) return 0;
     if (!(J.flags & JF_VEC4) || !(J.flags & JF_VEC4_ST)) return 0;
     if (J.post == POST_FOURSTEP && !(J.flags & JF_POST_LDS)) return 0;
     if (J.ncols % (1u << J.log_c) != 0) return 0;
     const uint32_t key = (J.log_r << 8) | J.log_c;
+    if (sin == Store::U32 && sout == Store::U32) {
+        switch (key) {
+#define NTT_SHAPE_KEY(LR, LC) case (LR << 8) | LC:
+            NTT_SHAPES(NTT_SHAPE_KEY)
+#undef NTT_SHAPE_KEY
+            return key;
+            default: return 0;
+        }
+    }
+    // uint64 data: matrix-core first passes (with post) and last passes
+    if (!(J.flags & JF_MFMA)) return 0;
+    if (sin == Store::U64 && (sout != Store::U32 || !(J.flags & JF_POST4))) return 0;
+    if (sin == Store::U32 && sout != Store::U64) return 0;
+    if (sin == Store::U64 && sout == Store::U64) return 0;
     switch (key) {
 #define NTT_SHAPE_KEY(LR, LC) case (LR << 8) | LC:
-        NTT_SHAPES(NTT_SHAPE_KEY)
+        NTT_SHAPES_U64(NTT_SHAPE_KEY)
 #undef NTT_SHAPE_KEY
         return key;
         default: return 0;

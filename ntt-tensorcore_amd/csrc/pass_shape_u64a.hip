@@ -1,0 +1,6 @@
+// Shape-specialised matrix-core pass kernels for uint64 data (first and last
+// passes), group A.
+#include "ntt_launch.hpp"
+namespace nttk {
+NTT_SHAPES_U64_A(NTT_SHAPE_U64_INSTANCE)
+}
