@@ -380,8 +380,8 @@ def tune_all(ctx, args):
     def rec(name, q, log_ns, elems):
         for r in nt.autotune(ctx, q, log_ns, elems, reps=4):
             out[f"{name}:2^{r['log_n']}"] = {k: r[k] for k in ("split", "log_e", "mfma_mask", "us", "heuristic_us")}
-    rec("single", BB, sorted({args.log_n, 12, 13, 14, 15, 16, 17, 18, 19, 20}), 1)
-    rec("batch_2^24", BB, [20, 16], 1 << 24)
+    rec("single", BB, sorted({args.log_n, *range(12, 24)}), 1)
+    rec("batch_2^24", BB, [23, 20, 16], 1 << 24)
     rec("rns", BB, [16], 24 << 16)
     rec("uneven", BB, list(range(10, 19)), 1 << 24)
     rec("serve_bb", BB, list(range(12, 23)), 1 << 24)
