@@ -840,7 +840,13 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
     // Predicates are needed for partial tiles and for tiles smaller than one sweep.
     const bool pred_l = !full || total < ((flags & JF_VEC4) ? 4u : 1u) * kNT;
     if (LR) {  // shape-specialised launches are full tiles with vector loads
-        load_loop<4, false, LR, LC, Sin, V>(J, poly, col0, pre);
+        // (the fused product's first pass exists only for uint64 data)
+        if constexpr (std::is_same<Sin, uint64_t>::value) {
+            if (flags & JF_MUL_IN) load_loop<4, false, LR, LC, Sin, V, true>(J, poly, col0, pre);
+            else load_loop<4, false, LR, LC, Sin, V>(J, poly, col0, pre);
+        } else {
+            load_loop<4, false, LR, LC, Sin, V>(J, poly, col0, pre);
+        }
     } else if (flags & JF_MUL_IN) {  // pointwise product fused into the load (predicated: rare path)
         if (flags & JF_VEC4) load_loop<4, true, LR, LC, Sin, V, true>(J, poly, col0, pre);
         else load_loop<1, true, LR, LC, Sin, V, true>(J, poly, col0, pre);

@@ -831,12 +831,13 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, co
 }
 
 // Shape-specialised kernel for job J, as (log_r << 8) | log_c, or 0: vector
-// loads and stores, full tiles, post tables in LDS, no fused product, and a
-// shape that has an instance for the job's storage (32-bit in and out, or a
-// matrix-core first or last pass on uint64 data).
+// loads and stores, full tiles, post tables in LDS, and a shape that has an
+// instance for the job's storage (32-bit in and out, or a matrix-core first
+// or last pass on uint64 data; the first may carry the fused product).
 static uint32_t pass_shape_of(const PassJob& J, Arith ar, Store sin, Store sout) {
     if (ar != Arith::M32) return 0;
-    if (J.flags & JF_MUL_IN) return 0;
+    // the fused product's load: uint64 first passes only (see below)
+    if ((J.flags & JF_MUL_IN) && sin != Store::U64) return 0;
     if (!(J.flags & JF_VEC4) || !(J.flags & JF_VEC4_ST)) return 0;
     if (J.post == POST_FOURSTEP && !(J.flags & JF_POST_LDS)) return 0;
     if (J.ncols % (1u << J.log_c) != 0) return 0;
