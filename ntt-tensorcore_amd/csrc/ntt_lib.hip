@@ -184,7 +184,9 @@ static Split heuristic_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight)
 
 // The tuned entry for (arith, log_n, elements in flight), else the tuned
 // entry for the same transform at the nearest number of elements in flight
-// (the split that won at another batch size), else the heuristic.
+// within a factor of 4 (a split that won at a similar load), else the
+// heuristic.  (A plan tuned for a 2^24-element batch can be much slower for
+// one small transform, whose few tiles cannot fill the chip.)
 static Split choose_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight) {
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -192,7 +194,7 @@ static Split choose_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight) {
         auto it = g_tuned.find(std::make_tuple((int)ar, log_n, le));
         if (it != g_tuned.end()) return it->second;
         const Split* best = nullptr;
-        uint32_t best_d = ~0u;
+        uint32_t best_d = 3;  // at most 2 apart in log2
         for (const auto& kv : g_tuned) {
             if (std::get<0>(kv.first) != (int)ar || std::get<1>(kv.first) != log_n) continue;
             const uint32_t e = std::get<2>(kv.first), d = e > le ? e - le : le - e;

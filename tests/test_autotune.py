@@ -27,18 +27,20 @@ def test_tune_table_load_save_roundtrip(nt, tmp_path):
 
 def test_split_lookup_falls_back_to_nearest_batch_size(nt, tmp_path):
     """An exact (arith, log n, elements in flight) entry wins; otherwise the
-    entry for the same n at the nearest load; otherwise the heuristic."""
+    entry for the same n at the nearest load within a factor of 4; otherwise
+    the heuristic."""
+    nt.autotune_clear()
+    heur = {(ln, le): nt.split_for(BB, ln, 1 << le) for ln, le in ((17, 17), (18, 15), (16, 24))}
     src = tmp_path / "t.txt"
     src.write_text("0 18 24 2 10 8 0 0 13 3\n0 18 20 2 9 9 0 0 12 0\n0 16 16 2 8 8 0 0 12 3\n")
-    nt.autotune_clear()
-    heur = nt.split_for(BB, 17, 1 << 17)
     nt.autotune_load(str(src))
     assert nt.split_for(BB, 18, 1 << 24) == {"split": [10, 8], "log_e": 13, "mfma_mask": 3}
     assert nt.split_for(BB, 18, 1 << 20) == {"split": [9, 9], "log_e": 12, "mfma_mask": 0}
     assert nt.split_for(BB, 18, 1 << 18) == {"split": [9, 9], "log_e": 12, "mfma_mask": 0}   # nearest: 2^20
     assert nt.split_for(BB, 18, 1 << 23) == {"split": [10, 8], "log_e": 13, "mfma_mask": 3}  # nearest: 2^24
-    assert nt.split_for(BB, 16, 1 << 24) == {"split": [8, 8], "log_e": 12, "mfma_mask": 3}
-    assert nt.split_for(BB, 17, 1 << 17) == heur                                             # no entry for n
+    assert nt.split_for(BB, 18, 1 << 15) == heur[(18, 15)]                                   # 2^20 is too far
+    assert nt.split_for(BB, 16, 1 << 24) == heur[(16, 24)]                                   # 2^16 is too far
+    assert nt.split_for(BB, 17, 1 << 17) == heur[(17, 17)]                                   # no entry for n
     nt.autotune_clear()
 
 
