@@ -386,6 +386,7 @@ def tune_all(ctx, args):
     rec("uneven", BB, list(range(10, 19)), 1 << 24)
     rec("serve_bb", BB, list(range(12, 23)), 1 << 24)
     rec("serve_62", W.p62_primes()[0], list(range(12, 23)), 1 << 24)
+    rec("single_62", W.p62_primes()[0], list(range(12, 23)), 1)
     return out
 
 
