@@ -19,11 +19,38 @@ def _residues(rng, mods, n):
 
 
 @pytest.mark.parametrize("L,K", [(1, 1), (3, 2), (8, 8), (24, 24), (32, 64), (5, 40)])
-@pytest.mark.parametrize("n", [1, 257, 1 << 16])
+@pytest.mark.parametrize("n", [1, 257, 1028, 1 << 16])
 def test_bconv_vs_oracle(nt, orc, L, K, n):
     primes = _primes(orc, L + K)
     q, p = primes[:L], primes[L:]
     x = _residues(np.random.default_rng(L * 1000 + K + n), q, n)
+    plan = nt.Bconv(q, p)
+    din = nt.DeviceBuffer.from_array(x)
+    dout = nt.DeviceBuffer(K * n * 4)
+    plan.run(dout, din, n)
+    nt.device_sync()
+    assert np.array_equal(dout.download(U32, K * n).reshape(K, n), orc.bconv(x, q, p))
+    plan.destroy()
+    din.free()
+    dout.free()
+
+
+@pytest.mark.parametrize("L,K,below,fill,n", [(32, 64, 1 << 31, "max", 4100), (32, 64, 1 << 31, "max", 4099),
+                                              (17, 20, 1 << 31, "zero", 4100), (20, 33, 1 << 18, "max", 4099),
+                                              (16, 16, 1 << 14, "random", 4100)])
+def test_bconv_moduli_sizes_and_extreme_residues(nt, orc, L, K, below, fill, n):
+    """Small target moduli (the centred matrix entries are then small) and
+    inputs at the ends of the range (every residue q_i - 1, or all zero);
+    n = 4100 runs the 16-byte access path with a partial last block, 4099 the
+    word path."""
+    primes = [int(v) for v in orc.ntt_primes(below, 1, L + K)]
+    q, p = primes[:L], primes[L:]
+    if fill == "max":
+        x = np.stack([np.full(n, m - 1, dtype=np.uint64) for m in q]).astype(U32)
+    elif fill == "zero":
+        x = np.zeros((L, n), dtype=U32)
+    else:
+        x = _residues(np.random.default_rng(L + K), q, n)
     plan = nt.Bconv(q, p)
     din = nt.DeviceBuffer.from_array(x)
     dout = nt.DeviceBuffer(K * n * 4)
