@@ -403,6 +403,8 @@ def main():
     ap.add_argument("--force-log-e", type=int, default=14)
     ap.add_argument("--force-mfma", type=int, default=0, help="mask of passes on the matrix cores")
     ap.add_argument("--tune-out", default="", help="write the tuned table here")
+    ap.add_argument("--extra", default="", help="run only this secondary measurement (profiling one kernel) "
+                                                "and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the NTT path runs on a single MI355X "
@@ -423,21 +425,31 @@ def main():
     else:
         nt.autotune_load(table)
         tuned = {"loaded": os.path.relpath(table, REPO)}
+    extra_fns = [("cyclic_4096", lambda: extra_latency_4096(ctx, stream)),
+                 ("reference_table", lambda: extra_reference_table(stream)),
+                 ("negacyclic_multiply", lambda: extra_polymul(stream)),
+                 ("negacyclic_2_20", lambda: extra_2_20(ctx, stream)),
+                 ("rns_24x2_16", lambda: extra_rns(ctx, stream)),
+                 ("rns_kernels", lambda: extra_rns_kernels(stream)),
+                 ("uneven_512", lambda: extra_uneven(ctx, stream)),
+                 ("u64_abi", lambda: extra_u64_abi(args, stream)),
+                 ("klimb", lambda: extra_klimb(stream)),
+                 ("serving", lambda: extra_serving(ctx))]
+    if args.extra:
+        fns = dict(extra_fns)
+        if args.extra not in fns:
+            print(f"unknown --extra {args.extra}; one of {', '.join(fns)}", file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps({"extra": args.extra, "result": fns[args.extra]()}), flush=True)
+        ctx.destroy()
+        stream.destroy()
+        return
     log(f"[bench] headline: negacyclic n=2^{args.log_n} fwd+inv, {args.warmup} warmup + {args.steps} steps")
     h = headline(args, ctx, stream)
     log(f"[bench] headline done: {h['value']:.1f} Gbfly/s, {h['roofline']['achieved']:.0f} GB/s")
     extras = {}
     if not args.no_extras:
-        for name, fn in [("cyclic_4096", lambda: extra_latency_4096(ctx, stream)),
-                         ("reference_table", lambda: extra_reference_table(stream)),
-                         ("negacyclic_multiply", lambda: extra_polymul(stream)),
-                         ("negacyclic_2_20", lambda: extra_2_20(ctx, stream)),
-                         ("rns_24x2_16", lambda: extra_rns(ctx, stream)),
-                         ("rns_kernels", lambda: extra_rns_kernels(stream)),
-                         ("uneven_512", lambda: extra_uneven(ctx, stream)),
-                         ("u64_abi", lambda: extra_u64_abi(args, stream)),
-                         ("klimb", lambda: extra_klimb(stream)),
-                         ("serving", lambda: extra_serving(ctx))]:
+        for name, fn in extra_fns:
             log(f"[bench] extra {name}")
             extras[name] = fn()
     cpu = None
