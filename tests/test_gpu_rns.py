@@ -64,10 +64,10 @@ def test_bconv_moduli_sizes_and_extreme_residues(nt, orc, L, K, below, fill, n):
 
 @pytest.mark.parametrize("dnum,L", [(1, 1), (3, 4), (8, 24), (32, 2)])
 @pytest.mark.parametrize("acc", [False, True])
-def test_pmac_vs_oracle(nt, orc, dnum, L, acc):
-    n = 3000
+@pytest.mark.parametrize("n", [3000, 3001])
+def test_pmac_vs_oracle(nt, orc, dnum, L, acc, n):
     q = _primes(orc, L)
-    rng = np.random.default_rng(dnum * 100 + L)
+    rng = np.random.default_rng(dnum * 100 + L + n)
     a = np.stack([_residues(rng, q, n) for _ in range(dnum)])
     b = np.stack([_residues(rng, q, n) for _ in range(dnum)])
     base = _residues(rng, q, n)
