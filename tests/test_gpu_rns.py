@@ -62,6 +62,24 @@ def test_bconv_moduli_sizes_and_extreme_residues(nt, orc, L, K, below, fill, n):
     dout.free()
 
 
+@pytest.mark.parametrize("L,K,n", [(3, 5, (1 << 21) + 260), (18, 2, (1 << 19) + 131)])
+def test_bconv_grid_stride(nt, orc, L, K, n):
+    """Sizes beyond one pass of the grid (8192 workgroups): the 16-byte path
+    (n % 4 == 0) and the word path each loop over further blocks."""
+    primes = _primes(orc, L + K)
+    q, p = primes[:L], primes[L:]
+    x = _residues(np.random.default_rng(n), q, n)
+    plan = nt.Bconv(q, p)
+    din = nt.DeviceBuffer.from_array(x)
+    dout = nt.DeviceBuffer(K * n * 4)
+    plan.run(dout, din, n)
+    nt.device_sync()
+    assert np.array_equal(dout.download(U32, K * n).reshape(K, n), orc.bconv(x, q, p))
+    plan.destroy()
+    din.free()
+    dout.free()
+
+
 @pytest.mark.parametrize("dnum,L", [(1, 1), (3, 4), (8, 24), (32, 2)])
 @pytest.mark.parametrize("acc", [False, True])
 @pytest.mark.parametrize("n", [3000, 3001])
