@@ -978,11 +978,11 @@ extern "C" ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out) {
                 if (!nega) return I->plan_status[k];  // cyclic must exist
             }
         }
-    // 2^10..2^13 with 32-bit arithmetic: [log_n/2, rest] with 2^10-element
-    // tiles on the VALU (at 2^12: 23.6 us per product against 28.5 us with the
-    // one-pass plan's three single-workgroup launches).  If it cannot be
-    // built, the product keeps plans[2].
-    if (I->plans[2] && I->plans[2]->npass == 1 && I->arith == Arith::M32 && log_n >= 10 && log_n <= 13) {
+    // 2^10..2^13: [log_n/2, rest] with 2^10-element tiles on the VALU.  At
+    // 2^12 a product takes 22.2 us against 28.5 us with the one-pass plan's
+    // three single-workgroup launches (BabyBear), 29.2 against 41.6 us with a
+    // 62-bit prime.  If it cannot be built, the product keeps plans[2].
+    if (I->plans[2] && I->plans[2]->npass == 1 && log_n >= 10 && log_n <= 13) {
         Split sp;
         sp.npass = 2;
         sp.log_r[0] = log_n / 2;

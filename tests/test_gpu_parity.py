@@ -182,7 +182,7 @@ def test_negacyclic_multiply_vs_schoolbook(nt, orc, cfgs, q, n):
 # The product runs fwd(a), fwd(b), then an inverse whose first pass loads
 # fwd(a)*fwd(b): sizes with one, two and three passes, 32- and 62-bit primes.
 @pytest.mark.parametrize("q,log_n", [(BB, 10), (BB, 11), (BB, 12), (BB, 13), (BB, 14), (BB, 16), (BB, 20),
-                                     (P30, 18), (P62, 11), (P62, 15)])
+                                     (P30, 18), (P62, 11), (P62, 12), (P62, 15)])
 @pytest.mark.parametrize("alias", [None, "a", "b"])
 def test_negacyclic_multiply_vs_oracle_pipeline(nt, orc, cfgs, q, log_n, alias):
     n = 1 << log_n
@@ -223,7 +223,7 @@ def test_negacyclic_square(nt, orc, cfgs, q, log_n):
 
 # a and b inside one allocation: the two forward transforms run as a batch
 # whose polynomial stride is b - a (even, odd or negative, in elements).
-@pytest.mark.parametrize("q,log_n", [(BB, 12), (BB, 14), (BB, 20), (P62, 13)])
+@pytest.mark.parametrize("q,log_n", [(BB, 12), (BB, 14), (BB, 20), (P62, 12), (P62, 13)])
 @pytest.mark.parametrize("gap", [0, 1, 6, -3])
 def test_negacyclic_multiply_strides(nt, orc, cfgs, q, log_n, gap):
     n = 1 << log_n
