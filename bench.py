@@ -84,8 +84,7 @@ def headline(args, ctx, stream):
     stream.sync()
     ok = np.array_equal(buf.download(np.uint32, n), x)
     buf.free()
-    if not ok:
-        raise RuntimeError("round trip after the timed region differs from the input")
+    _require(ok, "headline: the round trip after the timed region differs from the input")
     ms_step = ms / args.steps
     bfly = 2 * W.butterflies(log_n)
     t_transform_s = ms_step / 2 / 1e3
@@ -123,7 +122,8 @@ def cpu_baseline(args):
     return {"value": 2 * W.butterflies(args.log_n) / t / 1e9, "unit": "Gbutterfly/s", "cores": 1,
             "kind": "port",
             "sample": f"1 step: negacyclic forward+inverse round trip, n=2^{args.log_n}, oracle ntt_host "
-                      f"restatement (radix-2 DIT, __int128 %), one thread, {t:.2f} s"}
+                      f"restatement (radix-2 DIT, __int128 %), one thread (ntt_host is serial), {t:.2f} s",
+            **host_info()}
 
 
 def extra_latency_4096(ctx, stream):
@@ -206,8 +206,18 @@ def extra_2_20(ctx, stream):
         bt = nt.Batch(ctx, descs)
         ms = timed(lambda: bt.launch(stream), 50, 5, stream)
         t = ms / 50 / 1e3
+        # one more launch on fresh inputs, checked against the oracle
+        xs = [W.random_poly(n, BB, 100 + b) for b in range(batch)]
+        for b, x in enumerate(xs):
+            buf.upload(x.astype(np.uint32), b * n * 4)
+        bt.launch(stream)
+        stream.sync()
+        expect, _ = _oracle().transform_batch(xs, [BB] * batch, [True] * batch, [False] * batch)
+        ok = all(np.array_equal(buf.download(np.uint32, n, b * n * 4).astype(np.uint64), e)
+                 for b, e in enumerate(expect))
+        _require(ok, f"negacyclic_2_20 batch {batch}: output differs from the oracle")
         out[f"batch{batch}"] = {"Gbfly_s": batch * W.butterflies(20) / t / 1e9,
-                                "GB_s": batch * 2 * n * 4 / t / 1e9, "us": t * 1e6}
+                                "GB_s": batch * 2 * n * 4 / t / 1e9, "us": t * 1e6, "checked": "oracle"}
         bt.destroy()
         buf.free()
     return out
@@ -230,6 +240,90 @@ def _flags(j, u64=False):
             (nt.FLAG_U64 if u64 else 0))
 
 
+def _oracle():
+    """The CPU oracle, for the cpu_baseline legs: each extra that has a CPU
+    baseline times the oracle on the same inputs, and the oracle's outputs
+    double as the check of the GPU's."""
+    from oracle import pyoracle as orc
+    return orc
+
+
+class CheckFailed(RuntimeError):
+    pass
+
+
+def _require(ok, msg):
+    if not ok:
+        raise CheckFailed(msg)
+
+
+def _generator(q):
+    """The smallest g >= 2 with g^((q-1)/p) != 1 for each p in {2, 3, 5, 7}
+    dividing q - 1 (the generator rule of include/modular_arith.cuh:130-151)."""
+    g = 2
+    while any((q - 1) % p == 0 and pow(g, (q - 1) // p, q) == 1 for p in (2, 3, 5, 7)):
+        g += 1
+    return g
+
+
+def _sampled_ok(got, x, q, nega):
+    """A few outputs of a forward transform against the defining sums
+    out[k] = sum_j x_j r_k^j (r_k = psi^(2k+1) or w^k), evaluated in numpy."""
+    n = x.size
+    g = _generator(q)
+    for k in (0, 1, n // 3, n - 1):
+        r = pow(pow(g, (q - 1) // (2 * n), q), 2 * k + 1, q) if nega else pow(pow(g, (q - 1) // n, q), k, q)
+        p = np.ones(n, dtype=np.uint64)
+        m, rm = 1, r
+        while m < n:
+            c = min(m, n - m)
+            p[m:m + c] = (p[:c] * np.uint64(rm)) % np.uint64(q)
+            rm = rm * rm % q
+            m *= 2
+        if int(got[k]) != int(((x.astype(np.uint64) * p) % np.uint64(q)).sum(dtype=np.uint64) % np.uint64(q)):
+            return False
+    return True
+
+
+def _check_batch(buf, offs, jobs, run, stream):
+    """Re-uploads the seeded inputs of a packed batch, runs it once and
+    compares every polynomial with the oracle, whose transforms run one
+    polynomial per worker on all host cores.  Returns (oracle seconds, ok)."""
+    xs = [W.random_poly(1 << j["log_n"], j["q"], j["seed"]) for j in jobs]
+    for x, o in zip(xs, offs):
+        buf.upload(x.astype(np.uint32), o)
+    run()
+    stream.sync()
+    expect, t = _oracle().transform_batch(xs, [j["q"] for j in jobs], [j["nega"] for j in jobs],
+                                          [j["inverse"] for j in jobs])
+    ok = all(np.array_equal(buf.download(np.uint32, 1 << j["log_n"], o).astype(np.uint64), e)
+             for j, o, e in zip(jobs, offs, expect))
+    return t, ok
+
+
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return {"nproc": nproc, "cpu_model": model}
+
+
+def _cpu_pool(bfly, seconds, what):
+    threads = _oracle().host_threads()
+    return {"value": bfly / seconds / 1e9, "unit": "Gbutterfly/s", "cores": threads, "kind": "port",
+            "sample": f"{what}: oracle ntt_host restatement, one polynomial per worker on {threads} threads, "
+                      f"{seconds:.2f} s", **host_info()}
+
+
 def extra_rns(ctx, stream):
     primes = W.rns_primes()
     jobs = [{"log_n": 16, "q": q, "inverse": False, "nega": True, "seed": 0x5EED ^ i} for i, q in enumerate(primes)]
@@ -237,11 +331,15 @@ def extra_rns(ctx, stream):
     bt = nt.Batch(ctx, [(buf.ptr + o, j["q"], 16, _flags(j)) for j, o in zip(jobs, offs)])
     ms = timed(lambda: bt.launch_graph(stream), 50, 5, stream)
     t = ms / 50 / 1e3
+    cpu, ok = _check_batch(buf, offs, jobs, lambda: bt.launch_graph(stream), stream)
+    _require(ok, "rns_24x2_16: output differs from the oracle")
     bt.destroy()
     buf.free()
     n = 24 << 16
-    return {"limbs": 24, "log_n": 16, "us": t * 1e6, "Gbfly_s": 24 * W.butterflies(16) / t / 1e9,
-            "GB_s": 2 * n * 4 / t / 1e9, "frac": 2 * n * 4 / t / 1e9 / HBM_PEAK_GBS}
+    bfly = 24 * W.butterflies(16)
+    return {"limbs": 24, "log_n": 16, "us": t * 1e6, "Gbfly_s": bfly / t / 1e9,
+            "GB_s": 2 * n * 4 / t / 1e9, "frac": 2 * n * 4 / t / 1e9 / HBM_PEAK_GBS, "checked": "oracle",
+            "cpu_baseline": _cpu_pool(bfly, cpu, "24 limbs x 2^16 negacyclic forward")}
 
 
 def extra_rns_kernels(stream):
@@ -291,12 +389,17 @@ def extra_uneven(ctx, stream):
     bfly = sum(W.butterflies(j["log_n"]) for j in jobs)
     byt = sum(2 * (1 << j["log_n"]) * 4 for j in jobs)
     info = bt.info()
+    cpu, ok = _check_batch(buf, offs, jobs, lambda: bt.launch_graph(stream), stream)
+    _require(ok, "uneven_512: batched output differs from the oracle")
+    _, ok = _check_batch(buf, offs, jobs, per_poly, stream)
+    _require(ok, "uneven_512: per-polynomial output differs from the oracle")
     bt.destroy()
     buf.free()
     return {"polys": len(jobs), "batched_us": ms_b * 1e3, "per_poly_us": ms_p * 1e3,
             "speedup": ms_p / ms_b, "batched_Gbfly_s": bfly / (ms_b / 1e3) / 1e9,
             "batched_GB_s": byt / (ms_b / 1e3) / 1e9, "launches": info["launches"],
-            "per_poly_launches": sum(1 if j["log_n"] <= 14 else 2 for j in jobs)}
+            "per_poly_launches": sum(1 if j["log_n"] <= 14 else 2 for j in jobs), "checked": "oracle",
+            "cpu_baseline": _cpu_pool(bfly, cpu, "512 uneven polynomials, log2 n 10..18, negacyclic")}
 
 
 def extra_u64_abi(args, stream):
@@ -305,11 +408,19 @@ def extra_u64_abi(args, stream):
     cfg = nt.Config(n, BB)
     buf = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 24))
     ms = timed(lambda: (cfg.forward_negacyclic(buf, stream), cfg.inverse_negacyclic(buf, stream)), 10, 2, stream)
+    stream.sync()
+    x = W.random_poly(n, BB, 24)
+    ok = np.array_equal(buf.download(np.uint64, n), x)
+    _require(ok, "u64_abi: forward+inverse pairs do not round-trip")
+    cfg.forward_negacyclic(buf, stream)
+    stream.sync()
+    ok = _sampled_ok(buf.download(np.uint64, n), x, BB, nega=True)
+    _require(ok, "u64_abi: forward output differs from the defining sums")
     cfg.cleanup()
     buf.free()
     t = ms / 10 / 2 / 1e3
     return {"log_n": log_n, "Gbfly_s": W.butterflies(log_n) / t / 1e9, "GB_s": 2 * n * 8 / t / 1e9,
-            "frac": 2 * n * 8 / t / 1e9 / HBM_PEAK_GBS}
+            "frac": 2 * n * 8 / t / 1e9 / HBM_PEAK_GBS, "checked": "round trip + sampled defining sums"}
 
 
 def extra_klimb(stream):
@@ -346,15 +457,17 @@ def extra_serving(ctx, qps_list=(100, 500, 1000, 2000, 4000, 8000), count=120):
     total = sum((b + 255) // 256 * 256 for r in sizes for b in r)
     pool = nt.DeviceBuffer(total)
     nt.check(nt.lib().ntt_c_memset(pool.ptr, 1, total))  # 0x01.. < q for both prime kinds
-    reqs, off = [], 0
+    reqs, offs, off = [], [], 0
     for (_, polys), szs in zip(base, sizes):
-        r = []
+        r, ro = [], []
         for p, b in zip(polys, szs):
             u64 = p["q"] > (1 << 31)
             flags = nt.FLAG_NEGACYCLIC | (nt.FLAG_INVERSE if p["inverse"] else 0) | (nt.FLAG_U64 if u64 else 0)
             r.append((pool.ptr + off, p["q"], p["log_n"], flags))
+            ro.append(off)
             off += (b + 255) // 256 * 256
         reqs.append(r)
+        offs.append(ro)
     bfly = sum(W.butterflies(p["log_n"]) for _, polys in base for p in polys)
     srv = nt.Server(ctx, hbm_budget_bytes=16 << 30, max_inflight=4, max_batch_polys=1024)
     srv.replay(reqs, [0.0] * len(reqs))  # builds every plan once (twiddles resident)
@@ -368,6 +481,23 @@ def extra_serving(ctx, qps_list=(100, 500, 1000, 2000, 4000, 8000), count=120):
                               "p99_us": float(np.percentile(lat, 99)), "Gbfly_s": bfly / st["wall_s"] / 1e9,
                               "batches": st["batches"], "max_concurrent_batches": st["max_concurrent_batches"]})
         log(f"[bench]   qps={qps}: p50 {out['points'][-1]['p50_us']:.0f} us p99 {out['points'][-1]['p99_us']:.0f} us")
+    # check: seeded inputs for the requests of up to 2^21 coefficients, one
+    # more replay of those (all arriving at once, so they share batches), and
+    # every result against the oracle
+    pick = [i for i, (_, polys) in enumerate(base) if sum(1 << p["log_n"] for p in polys) <= (1 << 21)]
+    flat = [(p, o) for i in pick for p, o in zip(base[i][1], offs[i])]
+    xs = [W.random_poly(1 << p["log_n"], p["q"], p["seed"]) for p, _ in flat]
+    for (p, o), x in zip(flat, xs):
+        pool.upload(x.astype(np.uint64 if p["q"] > (1 << 31) else np.uint32), o)
+    srv.replay([reqs[i] for i in pick], [0.0] * len(pick))
+    expect, t_cpu = _oracle().transform_batch(xs, [p["q"] for p, _ in flat], [True] * len(flat),
+                                              [p["inverse"] for p, _ in flat])
+    ok = all(np.array_equal(pool.download(np.uint64 if p["q"] > (1 << 31) else np.uint32, 1 << p["log_n"], o)
+                            .astype(np.uint64), e) for (p, o), e in zip(flat, expect))
+    _require(ok, "serving: a result differs from the oracle")
+    out["checked"] = f"oracle, {len(pick)} requests ({len(flat)} polynomials) replayed together"
+    out["cpu_baseline"] = _cpu_pool(sum(W.butterflies(p["log_n"]) for p, _ in flat), t_cpu,
+                                    f"the {len(pick)} checked requests")
     srv.destroy()
     pool.free()
     return out
@@ -472,4 +602,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    try:
+        main()
+    except CheckFailed as e:
+        print(f"[bench] CHECK FAILED: {e}", file=sys.stderr, flush=True)
+        sys.exit(3)

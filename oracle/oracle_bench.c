@@ -71,3 +71,40 @@ double orc_time_batch(const uint64_t* ns, const uint64_t* qs, const int* dirs, i
     for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
     return now_s() - t0;
 }
+
+/* In-place transforms of `count` polynomials (data[i]: n[i] canonical
+ * residues mod q[i]; flags[i] bit 0 = inverse, bit 1 = negacyclic), one
+ * polynomial per worker on `threads` pthreads.  Each polynomial goes through
+ * the single-thread oracle (orc_ntt / orc_ntt_negacyclic), so the result is
+ * the oracle's whatever the thread count.  Returns wall seconds. */
+typedef struct {
+    uint64_t* const* data; const uint64_t* ns; const uint64_t* qs; const int* flags;
+    int count, next; pthread_mutex_t mu;
+} xform_t;
+void orc_ntt(uint64_t* data, uint64_t n, uint64_t q, int inverse);
+
+static void* xform_worker(void* arg) {
+    xform_t* J = (xform_t*)arg;
+    for (;;) {
+        pthread_mutex_lock(&J->mu);
+        int i = J->next++;
+        pthread_mutex_unlock(&J->mu);
+        if (i >= J->count) break;
+        const int inv = J->flags[i] & 1, nega = (J->flags[i] >> 1) & 1;
+        if (nega) orc_ntt_negacyclic(J->data[i], J->ns[i], J->qs[i], inv);
+        else orc_ntt(J->data[i], J->ns[i], J->qs[i], inv);
+    }
+    return NULL;
+}
+
+double orc_transform_batch(uint64_t* const* data, const uint64_t* ns, const uint64_t* qs, const int* flags,
+                           int count, int threads) {
+    xform_t J = {data, ns, qs, flags, count, 0, PTHREAD_MUTEX_INITIALIZER};
+    pthread_t th[256];
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    double t0 = now_s();
+    for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, xform_worker, &J);
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    return now_s() - t0;
+}

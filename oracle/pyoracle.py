@@ -70,6 +70,9 @@ def lib():
         L.orc_time_negacyclic_roundtrip.argtypes = [u64, u64, u64, ctypes.c_int]
         L.orc_time_batch.restype = ctypes.c_double
         L.orc_time_batch.argtypes = [U64P, U64P, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int]
+        L.orc_transform_batch.restype = ctypes.c_double
+        L.orc_transform_batch.argtypes = [ctypes.POINTER(U64P), U64P, U64P, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -238,3 +241,26 @@ def time_batch(ns, qs, dirs, threads):
     qs = np.ascontiguousarray(qs, dtype=np.uint64)
     d = (ctypes.c_int * len(dirs))(*[int(x) for x in dirs])
     return float(lib().orc_time_batch(_p64(ns), _p64(qs), d, len(dirs), threads))
+
+
+def host_threads():
+    """Worker threads for the oracle's batch transforms: the host's cores,
+    at most 16 (the GPU box's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def transform_batch(polys, qs, negas, invs, threads=None):
+    """Oracle transforms of many polynomials (copies), one polynomial per
+    worker thread; returns (results, wall seconds)."""
+    outs = [np.ascontiguousarray(p, dtype=np.uint64).copy() for p in polys]
+    k = len(outs)
+    ptrs = (U64P * k)(*[_p64(o) for o in outs])
+    ns = np.array([o.size for o in outs], dtype=np.uint64)
+    qa = np.array([int(q) for q in qs], dtype=np.uint64)
+    fl = (ctypes.c_int * k)(*[(1 if i else 0) | (2 if g else 0) for g, i in zip(negas, invs)])
+    t = float(lib().orc_transform_batch(ptrs, _p64(ns), _p64(qa), fl, k, threads or host_threads()))
+    return outs, t

@@ -11,6 +11,15 @@ pytestmark = pytest.mark.gpu
 BB = 2013265921
 
 
+@pytest.fixture(autouse=True)
+def _plan_table_cleared(nt):
+    """Every test starts and ends with an empty plan table, also when it fails
+    (a table a test installs must not steer the plans of later tests)."""
+    nt.autotune_clear()
+    yield
+    nt.autotune_clear()
+
+
 def _mfma_plans(nt, tmp_path, log_ns, elems):
     """Installs a plan table that runs every size of a batch of `elems`
     elements with radix-16 rounds on the matrix cores (the table keys plans

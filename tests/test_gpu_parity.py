@@ -55,7 +55,11 @@ def test_cfg1_babybear_4096_all_variants(nt, orc, cfgs, variant):
 def test_cyclic_vs_oracle(nt, orc, cfgs, log_n, q):
     n = 1 << log_n
     if (q - 1) % n:
-        pytest.skip("n does not divide q-1")
+        # no n-th root of unity: ntt_init refuses the modulus with a status
+        with pytest.raises(nt.NttError) as e:
+            nt.Config(n, q)
+        assert e.value.status == 2  # NTT_ERR_NO_ROOT
+        return
     c = cfgs(n, q)
     x = orc.random_poly(n, q, 1000 + log_n)
     assert np.array_equal(gpu_apply(nt, x, c.forward), orc.ntt(x, q))
@@ -67,7 +71,18 @@ def test_cyclic_vs_oracle(nt, orc, cfgs, log_n, q):
 def test_negacyclic_vs_oracle(nt, orc, cfgs, log_n, q):
     n = 1 << log_n
     if (q - 1) % (2 * n):
-        pytest.skip("2n does not divide q-1")
+        # no 2n-th root: the cyclic config may exist, the negacyclic transform
+        # is refused with a status and leaves the data untouched
+        if (q - 1) % n == 0:
+            x = orc.random_poly(n, q, 2000 + log_n)
+            buf = nt.DeviceBuffer.from_array(x)
+            with pytest.raises(nt.NttError) as e:
+                cfgs(n, q).forward_negacyclic(buf)
+            assert e.value.status == 2  # NTT_ERR_NO_ROOT
+            nt.device_sync()
+            assert np.array_equal(buf.download(U64, n), x)
+            buf.free()
+        return
     c = cfgs(n, q)
     x = orc.random_poly(n, q, 2000 + log_n)
     assert np.array_equal(gpu_apply(nt, x, c.forward_negacyclic), orc.ntt_negacyclic(x, q))

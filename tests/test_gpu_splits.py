@@ -4,6 +4,8 @@ VALU rounds only and with the radix-16 rounds on the matrix cores."""
 import numpy as np
 import pytest
 
+from props import sampled_sums_ok
+
 pytestmark = pytest.mark.gpu
 
 BB = 2013265921
@@ -46,11 +48,8 @@ def test_split_u32(nt, orc, ctx, log_n, split, log_e, kind, mf):
 
 @pytest.mark.parametrize("log_n,split,log_e", [(8, [4, 4], 8), (12, [6, 6], 10), (13, [5, 4, 4], 13),
                                                (16, [8, 8], 12), (18, [6, 6, 6], 12)])
-@pytest.mark.parametrize("q", [BB, P62])
-@pytest.mark.parametrize("mf", [False, True])
+@pytest.mark.parametrize("q,mf", [(BB, False), (BB, True), (P62, False)])  # 62-bit: VALU rounds only
 def test_split_u64(nt, orc, ctx, log_n, split, log_e, q, mf):
-    if mf and q == P62:
-        pytest.skip("62-bit arithmetic has no matrix-core rounds")
     n = 1 << log_n
     x = orc.random_poly(n, q, 3 * log_n)
     for flags, nega, inv in [(0, False, False), (3, True, True), (2, True, False)]:
@@ -83,12 +82,12 @@ MFMA_SPLITS = [
 @pytest.mark.parametrize("u64", [False, True])
 def test_mfma_split(nt, orc, ctx, log_n, split, log_e, mask, kind, u64):
     """Radix-16 rounds on the matrix cores in the passes of `mask`, against the
-    oracle, bit-exact; at n >= 2^22 through size-independent properties (round
-    trip through the VALU path, outputs sampled from the defining sums)."""
-    if log_n >= 22 and (u64 or kind in ("icyc", "inega")):
-        pytest.skip("one direction per kind at the largest sizes")
+    oracle, bit-exact; at n >= 2^22, in every direction and on both storage
+    widths, through size-independent properties (outputs sampled from the
+    defining sums, round trip through the heuristic plan)."""
     q = BB
     n = 1 << log_n
+    nega, inv = "nega" in kind, kind.startswith("i")
     flags = {"cyc": 0, "icyc": nt.FLAG_INVERSE, "nega": nt.FLAG_NEGACYCLIC,
              "inega": nt.FLAG_NEGACYCLIC | nt.FLAG_INVERSE}[kind] | (nt.FLAG_U64 if u64 else 0)
     dt = np.uint64 if u64 else np.uint32
@@ -98,34 +97,15 @@ def test_mfma_split(nt, orc, ctx, log_n, split, log_e, mask, kind, u64):
     nt.device_sync()
     got = buf.download(dt, n).astype(np.uint64)
     if log_n <= 20:
-        assert np.array_equal(got, orc.transform(x, q, "nega" in kind, kind.startswith("i"))), (split, mask)
+        assert np.array_equal(got, orc.transform(x, q, nega, inv)), (split, mask)
     else:
-        # sampled outputs from the defining sums, then the inverse through the VALU path
-        g = orc.generator(q)
-        if kind == "nega":
-            psi = pow(g, (q - 1) // (2 * n), q)
-            for k in (0, 3, n // 3, n - 1):
-                p = _pw(pow(psi, 2 * k + 1, q), n, q)
-                assert int(got[k]) == int(((x * p) % np.uint64(q)).sum(dtype=np.uint64) % np.uint64(q)), k
-        else:
-            w = pow(g, (q - 1) // n, q)
-            for k in (0, 5, n // 7, n - 1):
-                p = _pw(pow(w, k, q), n, q)
-                assert int(got[k]) == int(((x * p) % np.uint64(q)).sum(dtype=np.uint64) % np.uint64(q)), k
+        # sampled outputs from the defining sums, then the other direction
+        # through the heuristic plan (VALU rounds)
+        assert sampled_sums_ok(got, x, q, orc.generator(q), nega, inv, ks=(0, 3, n // 7, n - 1)), (split, mask)
         ctx.transform(buf.ptr, q, log_n, flags ^ nt.FLAG_INVERSE)
         nt.device_sync()
         assert np.array_equal(buf.download(dt, n).astype(np.uint64), x)
-
-
-def _pw(r, n, q):
-    p = np.ones(n, dtype=np.uint64)
-    m, rm = 1, r % q
-    while m < n:
-        k = min(m, n - m)
-        p[m:m + k] = (p[:k] * np.uint64(rm)) % np.uint64(q)
-        rm = rm * rm % q
-        m *= 2
-    return p
+    buf.free()
 
 
 @pytest.mark.parametrize("q,log_n,split,log_e", [(1056440321, 16, [8, 8], 13), (7340033, 15, [7, 8], 12),
