@@ -130,6 +130,7 @@ def extra_latency_4096(ctx, stream):
     """cfg1: n=2^12 cyclic forward + inverse through the uint64 ABI, one call pair per step."""
     n = 4096
     cfg = nt.Config(n, BB)
+    cfg.reserve(stream)
     buf = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 42))
     lat = []
     for i in range(60):
@@ -158,6 +159,7 @@ def extra_reference_table(stream):
     for log_n, ref_us in REFERENCE_FWD_US.items():
         n = 1 << log_n
         cfg = nt.Config(n, BB)
+        cfg.reserve(stream)
         buf = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 42))
         step = lambda: cfg.forward(buf, stream=stream)  # noqa: E731
         reps = [timed(step, 50, 5, stream) * 1e3 / 50 for _ in range(5)]
@@ -179,6 +181,7 @@ def extra_polymul(stream):
     for log_n in (12, 16, 20, 24):
         n = 1 << log_n
         cfg = nt.Config(n, BB)
+        cfg.reserve(stream)
         a = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 1))
         b = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 2))
         r = nt.DeviceBuffer(n * 8)
@@ -406,6 +409,7 @@ def extra_u64_abi(args, stream):
     log_n = args.log_n
     n = 1 << log_n
     cfg = nt.Config(n, BB)
+    cfg.reserve(stream)
     buf = nt.DeviceBuffer.from_array(W.random_poly(n, BB, 24))
     ms = timed(lambda: (cfg.forward_negacyclic(buf, stream), cfg.inverse_negacyclic(buf, stream)), 10, 2, stream)
     stream.sync()
@@ -544,7 +548,7 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     stream = nt.Stream()
-    ctx = nt.Context()
+    ctx = nt.Context(stream=stream)  # reserves the bench stream's workspace
     tuned = {}
     table = os.path.join(REPO, "ntt-tensorcore_amd", "tuned_mi355x.txt")
     if args.retune or not os.path.exists(table):

@@ -54,6 +54,13 @@ inline NTTConfig* ntt_init(uint64_t n, uint64_t q = DEFAULT_PRIME) {
 }
 // reference :38 / src/ntt_basic.cu:251-257
 inline void ntt_cleanup(NTTConfig* config) { check_(ntt_c_cleanup(config), "ntt_cleanup"); }
+// No reference counterpart: reserves the workspace of stream s (once, at
+// set-up) for calls of up to max_batch polynomials.  ntt_init reserves the
+// default stream's; calls on another stream without a reservation fail with
+// NTT_ERR_OOM instead of allocating.
+inline void ntt_reserve(const NTTConfig* c, hipStream_t s, uint64_t max_batch = 1) {
+    check_(ntt_c_config_reserve(c, s, max_batch), "ntt_reserve");
+}
 
 // reference :42-57
 inline void ntt_forward_basic(uint64_t* d, const NTTConfig* c, hipStream_t s = 0) {

@@ -5,8 +5,12 @@
  * entry point returns an ntt_status (0 = success) and never exits the
  * process.  Device pointers are HIP device memory; `stream` is a hipStream_t
  * passed as void* (NULL = the default stream).  All transforms are
- * asynchronous on that stream and allocate nothing on the transform path
- * (tables and workspace are made by the init/plan calls).
+ * asynchronous on that stream and allocate no device memory: tables are made
+ * by ntt_c_init / ntt_c_batch_prepare / ntt_c_context_prepare (or a context's
+ * first use of a (q, n, kind)), and the four-step workspace of a stream is
+ * reserved once at set-up (ntt_c_init: the NULL stream; ntt_c_config_reserve,
+ * ntt_c_context_reserve: any stream).  A transform on a stream without enough
+ * workspace returns NTT_ERR_OOM; it never allocates, synchronises or frees.
  *
  * Each entry point names the reference interface it replaces
  * (MMRROOO/ntt-tensorcore, file:line).  include/ntt.cuh and
@@ -69,11 +73,21 @@ enum {
 /* ---------------- 31-bit path: include/ntt.cuh ---------------- */
 
 /* ntt_init, reference src/ntt_basic.cu:217-249 (include/ntt.cuh:35).  Builds
- * params by the reference's generator rule, the device twiddle tables and the
- * transform plans.  q < 2^62 prime with n | q-1. */
+ * params by the reference's generator rule, the device twiddle tables, the
+ * transform plans and the NULL stream's workspace (one call of any kind; a
+ * batch runs in chunks that fit it).  q < 2^62 prime with n | q-1.  On
+ * failure nothing stays allocated. */
 ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out);
 /* ntt_cleanup, reference src/ntt_basic.cu:251-257 (include/ntt.cuh:38). */
 ntt_status ntt_c_cleanup(ntt_c_config* cfg);
+/* Set-up call with no reference counterpart (the reference's optimized path
+ * needs no workspace; its four-step allocates per call,
+ * src/ntt_four_step.cu:212-228): reserves the workspace of `stream` for calls
+ * of up to max_batch polynomials at once (larger batches run in chunks).
+ * Needed once per stream other than NULL before transforms on it; it may
+ * allocate and synchronise the stream, so it must not run while another
+ * thread issues transforms on that stream. */
+ntt_status ntt_c_config_reserve(const ntt_c_config* cfg, void* stream, uint64_t max_batch);
 
 /* ntt_forward_{basic,four_step,tlmop,optimized} / ntt_inverse_*, reference
  * include/ntt.cuh:42-57 (src/ntt_basic.cu:268-334, src/ntt_four_step.cu:244-392,
@@ -157,8 +171,20 @@ ntt_status ntt_c_context_create(ntt_c_context** out);
 ntt_status ntt_c_context_destroy(ntt_c_context* ctx);
 /* Bytes of device memory the context's tables use. */
 uint64_t ntt_c_context_table_bytes(const ntt_c_context* ctx);
+/* Set-up calls of the descriptor API (no reference counterpart):
+ * context_reserve makes the workspace of `stream` at least `bytes` (it may
+ * allocate and synchronise the stream; not while other threads issue work on
+ * it); context_prepare builds the plans of the descriptors' (q, n, kind) and
+ * reserves the workspace one of them needs on `stream`, so that later
+ * ntt_c_transform calls on them allocate nothing (graph capture, serving
+ * loops).  workspace_bytes: what one ntt_c_transform of *desc needs (at most
+ * 8 n bytes; 0 for one-pass sizes). */
+ntt_status ntt_c_context_reserve(ntt_c_context* ctx, void* stream, uint64_t bytes);
+ntt_status ntt_c_context_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count, void* stream);
+ntt_status ntt_c_workspace_bytes(const ntt_c_desc* desc, uint64_t* bytes);
 
-/* One transform, one polynomial, launched on its own (the per-polynomial path). */
+/* One transform, one polynomial, launched on its own (the per-polynomial
+ * path); its workspace is the stream's reservation (NTT_ERR_OOM if short). */
 ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* desc, void* stream);
 
 /* One transform with an explicit radix split n = 2^log_r[0] * ... * 2^log_r[npass-1]
@@ -175,8 +201,10 @@ ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc* desc, uin
  * builds tables for every (q, n) in the batch, the device job lists and the
  * workspace.  The descriptors' data pointers are bound at prepare time. */
 ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count, ntt_c_batch** out);
-/* Launch a prepared batch: at most three kernel launches whatever the mix
- * (single-pass polynomials, then the two passes of the four-step ones). */
+/* Launch a prepared batch: one kernel launch per (pass, arithmetic, storage
+ * pair, round kind, post-multiply kind) group of its jobs, so the launch
+ * count depends on the kinds in the mix, not on the number of polynomials
+ * (ntt_c_batch_info reports it). */
 ntt_status ntt_c_batch_launch(ntt_c_batch* batch, void* stream);
 /* Capture the batch's launches into a hipGraph once; later calls replay it. */
 ntt_status ntt_c_batch_launch_graph(ntt_c_batch* batch, void* stream);
@@ -313,6 +341,13 @@ ntt_status ntt_c_event_create(void** ev);
 ntt_status ntt_c_event_destroy(void* ev);
 ntt_status ntt_c_event_record(void* ev, void* stream);
 ntt_status ntt_c_event_elapsed_ms(float* ms, void* start, void* end);
+/* Stream capture into a hipGraph (global capture mode, so an allocation or
+ * synchronisation inside the captured calls fails the capture), instantiate,
+ * replay, destroy. */
+ntt_status ntt_c_graph_begin(void* stream);
+ntt_status ntt_c_graph_end(void* stream, void** graph_exec);
+ntt_status ntt_c_graph_launch(void* graph_exec, void* stream);
+ntt_status ntt_c_graph_destroy(void* graph_exec);
 /* Human-readable text of the last error on this thread. */
 const char* ntt_c_last_error(void);
 

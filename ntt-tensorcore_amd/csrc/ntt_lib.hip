@@ -647,11 +647,11 @@ static void set_shifts(PassJob& J) {
 }
 
 // Fills the PassJobs of `npoly` polynomials of plan P at data (stride n
-// elements, or dstride with two or more passes), with workspace at `scratch`
+// elements, or dstride with two or more passes), with workspace at `work`
 // (stride n, arithmetic width).  With mul_a, the first pass reads
 // mul_a[i]*mul_b[i] mod q (canonical residues, same layout and storage as
 // data) instead of data.
-static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, PassJob out[4],
+static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void* work, PassJob out[4],
                       Store sin_out[4][2], const void* mul_a = nullptr, const void* mul_b = nullptr,
                       int64_t dstride = 0) {
     const uint32_t log_n = P.log_n;
@@ -659,8 +659,8 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
     if (dstride == 0) dstride = N;
     const char* blk = (const char*)P.block;
     const Store mid = P.arith == Arith::M32 ? Store::U32 : Store::U64;
-    // second 32-bit scratch region (after npoly polynomials of the first)
-    void* const s2 = two_scratch(P, st) ? (char*)scratch + ((npoly << log_n) << 2) : nullptr;
+    // second 32-bit workspace region (after npoly polynomials of the first)
+    void* const s2 = two_scratch(P, st) ? (char*)work + ((npoly << log_n) << 2) : nullptr;
     void* src = data;
     Store src_st = st;
     for (int i = 0; i < P.npass; i++) {
@@ -686,8 +686,8 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         void* dst;
         Store dst_st;
         if (last) dst = data, dst_st = st;
-        else if (s2) dst = src == scratch ? s2 : scratch, dst_st = mid;
-        else dst = src == data ? scratch : data, dst_st = src == data ? mid : st;
+        else if (s2) dst = src == work ? s2 : work, dst_st = mid;
+        else dst = src == data ? work : data, dst_st = src == data ? mid : st;
         J.in = src;
         J.out = dst;
         if (i == 0 && mul_a) {
@@ -878,11 +878,11 @@ static uint32_t group_shape(const PassJob* jobs, size_t n, Arith ar, Store sin, 
 // Runs plan P on npoly polynomials at data (stride n, or dstride); with
 // mul_a, on mul_a*mul_b mod q elementwise, the result going to data
 // (make_jobs).
-static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, void* scratch, hipStream_t stream,
+static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, void* work, hipStream_t stream,
                            const void* mul_a = nullptr, const void* mul_b = nullptr, int64_t dstride = 0) {
     PassJob jobs[4];
     Store io[4][2];
-    make_jobs(P, data, st, npoly, scratch, jobs, io, mul_a, mul_b, dstride);
+    make_jobs(P, data, st, npoly, work, jobs, io, mul_a, mul_b, dstride);
     for (int i = 0; i < P.npass; i++) {
         ntt_status s = launch_pass(P.arith, io[i][0], io[i][1], job_kind(jobs[i]), nullptr, 0, jobs[i],
                                    job_tiles(jobs[i]), job_lds(jobs[i], P.arith), stream,
@@ -893,6 +893,53 @@ static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, 
 }
 
 // ------------------------------------------------------------------ config (include/ntt.cuh)
+// Device workspace of one stream: allocated by a set-up call (ntt_c_init for
+// the NULL stream, ntt_c_config_reserve / ntt_c_context_reserve for others)
+// and never resized by a transform, so the transform path allocates nothing.
+struct Workspace {
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+};
+
+// Replaces the workspace of `stream` by one of at least `bytes` (a set-up
+// call: it waits for the stream before freeing a smaller buffer).
+static ntt_status reserve_workspace(std::mutex& mu, std::map<hipStream_t, Workspace>& work, hipStream_t stream,
+                                    uint64_t bytes) {
+    std::lock_guard<std::mutex> lk(mu);
+    Workspace& w = work[stream];
+    if (w.bytes >= bytes) return NTT_OK;
+    if (w.ptr) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        HIP_TRY(hipFree(w.ptr));
+        w.ptr = nullptr;
+        w.bytes = 0;
+    }
+    if (hipMalloc(&w.ptr, bytes) != hipSuccess) return fail(NTT_ERR_OOM, "workspace allocation failed");
+    w.bytes = bytes;
+    return NTT_OK;
+}
+
+// The workspace of `stream`; NTT_ERR_OOM if it holds fewer than `need` bytes
+// (nothing was reserved for the stream, or less than this call needs).
+static ntt_status find_workspace(std::mutex& mu, std::map<hipStream_t, Workspace>& work, hipStream_t stream,
+                                 uint64_t need, Workspace* out) {
+    *out = Workspace{};
+    if (need == 0) return NTT_OK;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = work.find(stream);
+    if (it == work.end() || it->second.bytes < need)
+        return fail(NTT_ERR_OOM, "no workspace of " + std::to_string(need) +
+                                     " bytes reserved for this stream (reserve it once at set-up)");
+    *out = it->second;
+    return NTT_OK;
+}
+
+static void free_workspaces(std::map<hipStream_t, Workspace>& work) {
+    for (auto& kv : work)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    work.clear();
+}
+
 struct ConfigImpl {
     std::unique_ptr<Plan> plans[4];  // [nega*2 + inv]
     // Negacyclic forward plan of ntt_c_negacyclic_multiply where plans[2] is a
@@ -902,35 +949,12 @@ struct ConfigImpl {
     ntt_status plan_status[4] = {NTT_OK, NTT_OK, NTT_OK, NTT_OK};
     std::string plan_error[4];
     Arith arith = Arith::M32;
+    uint64_t ws_poly = 0;  // workspace bytes per polynomial of the largest plan
+    uint64_t ws_call = 0;  // workspace bytes of the largest single-polynomial call
     std::mutex mu;
-    std::map<hipStream_t, std::pair<void*, uint64_t>> scratch;  // per stream: ptr, bytes
-    ~ConfigImpl() {
-        for (auto& kv : scratch)
-            if (kv.second.first) (void)hipFree(kv.second.first);
-    }
+    std::map<hipStream_t, Workspace> work;  // per stream
+    ~ConfigImpl() { free_workspaces(work); }
 };
-
-// Workspace for multi-pass transforms, one buffer per stream so that one config
-// can serve several streams at once; grown on demand, never shrunk.
-static ntt_status scratch_for(ConfigImpl* I, hipStream_t stream, uint64_t bytes, void** out) {
-    *out = nullptr;
-    if (bytes == 0) return NTT_OK;
-    std::lock_guard<std::mutex> lk(I->mu);
-    auto& ent = I->scratch[stream];
-    if (ent.second < bytes) {
-        if (ent.first) {
-            HIP_TRY(hipStreamSynchronize(stream));
-            HIP_TRY(hipFree(ent.first));
-            ent.first = nullptr;
-            ent.second = 0;
-        }
-        hipError_t e = hipMalloc(&ent.first, bytes);
-        if (e != hipSuccess) return fail(NTT_ERR_OOM, "hipMalloc(scratch) failed");
-        ent.second = bytes;
-    }
-    *out = ent.first;
-    return NTT_OK;
-}
 
 extern "C" uint64_t ntt_c_find_primitive_root(uint64_t q, uint64_t n) {
     if (q < 3 || n == 0) return 0;
@@ -952,23 +976,13 @@ extern "C" ntt_status ntt_c_create_params(uint64_t q, uint64_t n, ntt_c_modular_
     return NTT_OK;
 }
 
-extern "C" ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out) {
-    if (!out) return fail(NTT_ERR_INVALID_ARG, "ntt_init: null output");
-    *out = nullptr;
-    if (n < 2 || (n & (n - 1))) return fail(NTT_ERR_INVALID_ARG, "ntt_init: n must be a power of two >= 2");
-    const uint32_t log_n = (uint32_t)__builtin_ctzll(n);
-    ntt_status st = check_modulus(q, log_n, false);
-    if (st != NTT_OK) return st;
-    if (log_n > 26) return fail(NTT_ERR_UNSUPPORTED, "ntt_init: n > 2^26");
-    auto C = std::make_unique<ntt_c_config>();
-    memset(C.get(), 0, sizeof(ntt_c_config));
-    C->n = n;
-    C->log_n = log_n;
-    C->q = q;
-    C->mu = (uint64_t)(((u128)1 << 64) / q);
-    ntt_c_create_params(q, n, &C->params);
-    auto I = std::make_unique<ConfigImpl>();
-    I->arith = q < (1ull << 31) ? Arith::M32 : Arith::M64;
+extern "C" ntt_status ntt_c_cleanup(ntt_c_config* cfg);
+
+// The config's tables and the NULL stream's workspace are made here; no
+// transform call allocates.  On failure everything built so far is freed.
+static ntt_status init_config(ntt_c_config* C, ConfigImpl* I) {
+    const uint64_t n = C->n, q = C->q;
+    const uint32_t log_n = (uint32_t)C->log_n;
     for (int nega = 0; nega < 2; nega++)
         for (int inv = 0; inv < 2; inv++) {
             const int k = nega * 2 + inv;
@@ -991,6 +1005,17 @@ extern "C" ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out) {
         sp.mfma = 0;
         if (build_plan(q, log_n, true, false, I->arith, sp, I->prod_fwd) != NTT_OK) I->prod_fwd.reset();
     }
+    // Workspace: per polynomial, the largest plan's; per call, also the
+    // product's two-operand forward batch and the rowmaj layout copy.
+    for (auto& P : I->plans)
+        if (P) I->ws_poly = std::max(I->ws_poly, plan_scratch_elems(*P, Store::U64) * vbytes(P->arith));
+    I->ws_call = I->ws_poly;
+    if (I->prod_fwd)
+        I->ws_call = std::max(I->ws_call, 2 * plan_scratch_elems(*I->prod_fwd, Store::U64) * vbytes(I->arith));
+    I->ws_call = std::max(I->ws_call, n * sizeof(uint64_t) +
+                                          plan_scratch_elems(*I->plans[0], Store::U64) * vbytes(I->arith));
+    ntt_status st = reserve_workspace(I->mu, I->work, nullptr, I->ws_call);
+    if (st != NTT_OK) return st;
     // d_twiddles[k] = omega^k, d_twiddles_inv[k] = omega^-k (reference src/ntt_basic.cu:233-246),
     // generated on the device from two sqrt(n)-entry tables.
     HIP_TRY(hipMalloc(&C->d_twiddles, n * sizeof(uint64_t)));
@@ -1001,23 +1026,50 @@ extern "C" ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out) {
         TableBuilder tb{I->arith, q, {}};
         const int lo = tb.add(powers(w, 1ull << bits, q));
         const int hi = tb.add(powers(powmod(w, 1ull << bits, q), (n >> bits) + 1, q));
-        void* dt = nullptr;
-        HIP_TRY(hipMalloc(&dt, tb.host.size()));
-        HIP_TRY(hipMemcpy(dt, tb.host.data(), tb.host.size(), hipMemcpyHostToDevice));
+        std::unique_ptr<void, void (*)(void*)> dt(nullptr, [](void* p) { if (p) (void)hipFree(p); });
+        void* raw = nullptr;
+        HIP_TRY(hipMalloc(&raw, tb.host.size()));
+        dt.reset(raw);
+        HIP_TRY(hipMemcpy(raw, tb.host.data(), tb.host.size(), hipMemcpyHostToDevice));
         uint64_t* dst = inv ? C->d_twiddles_inv : C->d_twiddles;
         const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 4096);
         if (I->arith == Arith::M32)
             hipLaunchKernelGGL(powers_kernel<Mod32>, dim3(blocks), dim3(256), 0, 0, dst, n,
-                               (const TwP<uint32_t>*)((char*)dt + lo), (const TwP<uint32_t>*)((char*)dt + hi), bits, q);
+                               (const TwP<uint32_t>*)((char*)raw + lo), (const TwP<uint32_t>*)((char*)raw + hi), bits, q);
         else
             hipLaunchKernelGGL(powers_kernel<Mod64>, dim3(blocks), dim3(256), 0, 0, dst, n,
-                               (const TwP<uint64_t>*)((char*)dt + lo), (const TwP<uint64_t>*)((char*)dt + hi), bits, q);
+                               (const TwP<uint64_t>*)((char*)raw + lo), (const TwP<uint64_t>*)((char*)raw + hi), bits, q);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipDeviceSynchronize());
-        HIP_TRY(hipFree(dt));
     }
-    C->impl = I.release();
-    *out = C.release();
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out) {
+    if (!out) return fail(NTT_ERR_INVALID_ARG, "ntt_init: null output");
+    *out = nullptr;
+    if (n < 2 || (n & (n - 1))) return fail(NTT_ERR_INVALID_ARG, "ntt_init: n must be a power of two >= 2");
+    const uint32_t log_n = (uint32_t)__builtin_ctzll(n);
+    ntt_status st = check_modulus(q, log_n, false);
+    if (st != NTT_OK) return st;
+    if (log_n > 26) return fail(NTT_ERR_UNSUPPORTED, "ntt_init: n > 2^26");
+    ntt_c_config* C = new ntt_c_config();
+    memset(C, 0, sizeof(ntt_c_config));
+    C->n = n;
+    C->log_n = log_n;
+    C->q = q;
+    C->mu = (uint64_t)(((u128)1 << 64) / q);
+    ntt_c_create_params(q, n, &C->params);
+    ConfigImpl* I = new ConfigImpl();
+    I->arith = q < (1ull << 31) ? Arith::M32 : Arith::M64;
+    C->impl = I;
+    st = init_config(C, I);
+    if (st != NTT_OK) {
+        const std::string msg = g_last_error;
+        (void)ntt_c_cleanup(C);
+        return fail(st, msg);
+    }
+    *out = C;
     return NTT_OK;
 }
 
@@ -1031,6 +1083,18 @@ extern "C" ntt_status ntt_c_cleanup(ntt_c_config* cfg) {
     return NTT_OK;
 }
 
+extern "C" ntt_status ntt_c_config_reserve(const ntt_c_config* cfg, void* stream, uint64_t max_batch) {
+    if (!cfg || !cfg->impl) return fail(NTT_ERR_INVALID_ARG, "config_reserve: null config");
+    ConfigImpl* I = (ConfigImpl*)cfg->impl;
+    const uint64_t b = std::max<uint64_t>(max_batch, 1);
+    if (I->ws_poly && b > (~0ull >> 1) / I->ws_poly) return fail(NTT_ERR_INVALID_ARG, "config_reserve: batch too large");
+    return reserve_workspace(I->mu, I->work, (hipStream_t)stream, std::max(I->ws_call, I->ws_poly * b));
+}
+
+// Runs a config plan on `batch` polynomials at d.  The batch runs in chunks
+// of as many polynomials as the stream's workspace holds (at least one: the
+// reserve calls size it for one call of every kind), one after another on
+// the stream, so a batch larger than the reservation needs no allocation.
 static ntt_status run_config(uint64_t* d, const ntt_c_config* cfg, bool nega, bool inv, uint64_t batch,
                              void* stream, const uint64_t* mul_a = nullptr, const uint64_t* mul_b = nullptr) {
     if (!d || !cfg || !cfg->impl) return fail(NTT_ERR_INVALID_ARG, "null data or config");
@@ -1039,11 +1103,15 @@ static ntt_status run_config(uint64_t* d, const ntt_c_config* cfg, bool nega, bo
     const int k = (nega ? 2 : 0) + (inv ? 1 : 0);
     if (!I->plans[k]) return fail(I->plan_status[k], I->plan_error[k]);
     const Plan& P = *I->plans[k];
-    void* scratch = nullptr;
-    const uint64_t sbytes = plan_scratch_elems(P, Store::U64) * batch * vbytes(P.arith);
-    ntt_status st = scratch_for(I, (hipStream_t)stream, sbytes, &scratch);
+    const uint64_t per = plan_scratch_elems(P, Store::U64) * vbytes(P.arith);
+    Workspace ws;
+    ntt_status st = find_workspace(I->mu, I->work, (hipStream_t)stream, per, &ws);
     if (st != NTT_OK) return st;
-    return run_plan(P, d, Store::U64, batch, scratch, (hipStream_t)stream, mul_a, mul_b);
+    const uint64_t chunk = per ? std::max<uint64_t>(1, ws.bytes / per) : batch;
+    for (uint64_t b0 = 0; b0 < batch && st == NTT_OK; b0 += chunk)
+        st = run_plan(P, d + b0 * cfg->n, Store::U64, std::min(chunk, batch - b0), ws.ptr, (hipStream_t)stream,
+                      mul_a, mul_b);
+    return st;
 }
 
 static bool valid_variant(int v) { return v >= NTT_VARIANT_BASIC && v <= NTT_VARIANT_FULL_OPTIMIZED; }
@@ -1109,9 +1177,10 @@ extern "C" ntt_status ntt_c_negacyclic_multiply(uint64_t* r, uint64_t* a, uint64
     } else if (P.npass >= 2) {
         // both forward transforms in one launch per pass: a batch of two
         // polynomials whose stride is b - a
-        void* scratch = nullptr;
-        st = scratch_for(I, (hipStream_t)stream, plan_scratch_elems(P, Store::U64) * 2 * vbytes(P.arith), &scratch);
-        if (st == NTT_OK) st = run_plan(P, a, Store::U64, 2, scratch, (hipStream_t)stream, nullptr, nullptr, d);
+        Workspace ws;
+        st = find_workspace(I->mu, I->work, (hipStream_t)stream, plan_scratch_elems(P, Store::U64) * 2 * vbytes(P.arith),
+                            &ws);
+        if (st == NTT_OK) st = run_plan(P, a, Store::U64, 2, ws.ptr, (hipStream_t)stream, nullptr, nullptr, d);
     } else {
         st = ntt_c_forward_negacyclic(a, cfg, stream);
         if (st == NTT_OK) st = ntt_c_forward_negacyclic(b, cfg, stream);
@@ -1189,12 +1258,12 @@ extern "C" ntt_status ntt_c_forward_rowmaj(uint64_t* d, const ntt_c_config* cfg,
     const Plan& P = *I->plans[0];
     const uint64_t tbytes = cfg->n * sizeof(uint64_t);
     const uint64_t sbytes = plan_scratch_elems(P, Store::U64) * vbytes(P.arith);
-    void* ws = nullptr;
-    ntt_status st = scratch_for(I, (hipStream_t)stream, tbytes + sbytes, &ws);
+    Workspace ws;
+    ntt_status st = find_workspace(I->mu, I->work, (hipStream_t)stream, tbytes + sbytes, &ws);
     if (st != NTT_OK) return st;
-    uint64_t* tmp = (uint64_t*)ws;
+    uint64_t* tmp = (uint64_t*)ws.ptr;
     st = transpose(tmp, d, n2, n1, (hipStream_t)stream);  // pre [N2][N1] -> canon
-    if (st == NTT_OK) st = run_plan(P, tmp, Store::U64, 1, (char*)ws + tbytes, (hipStream_t)stream);
+    if (st == NTT_OK) st = run_plan(P, tmp, Store::U64, 1, (char*)ws.ptr + tbytes, (hipStream_t)stream);
     if (st == NTT_OK) st = transpose(d, tmp, n1, n2, (hipStream_t)stream);  // canon -> pre
     return st;
 }
@@ -1233,11 +1302,8 @@ struct ntt_c_context {
     std::map<std::tuple<uint64_t, uint32_t, int, int, int, Split>, std::unique_ptr<Plan>> plans;
     uint64_t table_bytes = 0;
     std::mutex smu;
-    std::map<hipStream_t, std::pair<void*, uint64_t>> scratch;  // per-call four-step workspace per stream
-    ~ntt_c_context() {
-        for (auto& kv : scratch)
-            if (kv.second.first) (void)hipFree(kv.second.first);
-    }
+    std::map<hipStream_t, Workspace> work;  // per stream, from ntt_c_context_reserve
+    ~ntt_c_context() { free_workspaces(work); }
 };
 
 static Arith arith_for(uint64_t q, uint32_t flags) {
@@ -1301,8 +1367,8 @@ struct ntt_c_batch {
         PassJob inl{};
     };
     std::vector<Group> phases[4];  // phase k: pass k of every polynomial
-    void* scratch = nullptr;
-    uint64_t scratch_bytes = 0;
+    void* work = nullptr;
+    uint64_t work_bytes = 0;
     uint64_t workgroups = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1312,7 +1378,7 @@ struct ntt_c_batch {
         for (auto& ph : phases)
             for (auto& g : ph)
                 if (g.djobs) (void)hipFree(g.djobs);
-        if (scratch) (void)hipFree(scratch);
+        if (work) (void)hipFree(work);
     }
 };
 
@@ -1324,7 +1390,7 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
     struct Item { PassJob job; Arith ar; Store sin, sout; int phase; };
     std::vector<Item> items;
     std::vector<const Plan*> plans(count);
-    uint64_t scratch_bytes = 0, elems = 0;
+    uint64_t work_bytes = 0, elems = 0;
     for (size_t i = 0; i < count; i++) {
         ntt_status st = check_desc(descs[i]);
         if (st != NTT_OK) return st;
@@ -1334,12 +1400,12 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
         ntt_status st = ctx_plan(ctx, descs[i], elems, &plans[i]);
         if (st != NTT_OK) return st;
         const Store sd = (descs[i].flags & NTT_FLAG_U64) ? Store::U64 : Store::U32;
-        scratch_bytes += ((plan_scratch_elems(*plans[i], sd) * vbytes(plans[i]->arith)) + 255) & ~255ull;
+        work_bytes += ((plan_scratch_elems(*plans[i], sd) * vbytes(plans[i]->arith)) + 255) & ~255ull;
     }
-    if (scratch_bytes) {
-        hipError_t e = hipMalloc(&B->scratch, scratch_bytes);
-        if (e != hipSuccess) return fail(NTT_ERR_OOM, "batch_prepare: scratch allocation failed");
-        B->scratch_bytes = scratch_bytes;
+    if (work_bytes) {
+        hipError_t e = hipMalloc(&B->work, work_bytes);
+        if (e != hipSuccess) return fail(NTT_ERR_OOM, "batch_prepare: workspace allocation failed");
+        B->work_bytes = work_bytes;
     }
     uint64_t soff = 0;
     for (size_t i = 0; i < count; i++) {
@@ -1347,7 +1413,7 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
         const Store st = (descs[i].flags & NTT_FLAG_U64) ? Store::U64 : Store::U32;
         void* scr = nullptr;
         if (P.npass >= 2) {
-            scr = (char*)B->scratch + soff;
+            scr = (char*)B->work + soff;
             soff += ((plan_scratch_elems(P, st) * vbytes(P.arith)) + 255) & ~255ull;
         }
         PassJob jobs[4];
@@ -1451,27 +1517,46 @@ extern "C" ntt_status ntt_c_batch_info(const ntt_c_batch* B, uint32_t* launches,
         for (auto& ph : B->phases) l += (uint32_t)ph.size();
         *launches = l;
     }
-    if (ws) *ws = B->scratch_bytes;
+    if (ws) *ws = B->work_bytes;
     if (wgs) *wgs = B->workgroups;
     return NTT_OK;
 }
 
-static ntt_status ctx_scratch(ntt_c_context* ctx, hipStream_t stream, uint64_t need, void** out) {
-    *out = nullptr;
-    if (!need) return NTT_OK;
-    std::lock_guard<std::mutex> lk(ctx->smu);
-    auto& ent = ctx->scratch[stream];
-    if (ent.second < need) {
-        if (ent.first) {
-            HIP_TRY(hipStreamSynchronize(stream));
-            HIP_TRY(hipFree(ent.first));
-        }
-        ent.first = nullptr;
-        ent.second = 0;
-        if (hipMalloc(&ent.first, need) != hipSuccess) return fail(NTT_ERR_OOM, "transform: scratch allocation failed");
-        ent.second = need;
+extern "C" ntt_status ntt_c_context_reserve(ntt_c_context* ctx, void* stream, uint64_t bytes) {
+    if (!ctx) return fail(NTT_ERR_INVALID_ARG, "context_reserve: null context");
+    if (bytes == 0) return NTT_OK;
+    return reserve_workspace(ctx->smu, ctx->work, (hipStream_t)stream, bytes);
+}
+
+// Workspace bytes of one transform of the descriptor's kind (the split it
+// would get now; 0 for one-pass plans).
+static uint64_t desc_workspace(const ntt_c_desc& d, const Split& sp) {
+    if (sp.npass < 2) return 0;
+    const Arith ar = arith_for(d.q, d.flags);
+    const bool u64 = d.flags & NTT_FLAG_U64;
+    const bool two = ar == Arith::M32 && u64 && sp.npass >= 3;
+    return ((1ull << d.log_n) << (two ? 1 : 0)) * vbytes(ar);
+}
+
+extern "C" ntt_status ntt_c_context_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count, void* stream) {
+    if (!ctx || (count && !descs)) return fail(NTT_ERR_INVALID_ARG, "context_prepare: null argument");
+    uint64_t need = 0;
+    for (size_t i = 0; i < count; i++) {
+        ntt_status st = check_desc(descs[i]);
+        if (st != NTT_OK) return st;
+        const Plan* P = nullptr;
+        st = ctx_plan(ctx, descs[i], 1ull << descs[i].log_n, &P);
+        if (st != NTT_OK) return st;
+        need = std::max(need, desc_workspace(descs[i], P->split));
     }
-    *out = ent.first;
+    return ntt_c_context_reserve(ctx, stream, need);
+}
+
+extern "C" ntt_status ntt_c_workspace_bytes(const ntt_c_desc* d, uint64_t* bytes) {
+    if (!d || !bytes) return fail(NTT_ERR_INVALID_ARG, "workspace_bytes: null argument");
+    ntt_status st = check_desc(*d);
+    if (st != NTT_OK) return st;
+    *bytes = desc_workspace(*d, choose_split(d->log_n, arith_for(d->q, d->flags), 1ull << d->log_n));
     return NTT_OK;
 }
 
@@ -1482,13 +1567,12 @@ static ntt_status transform_one(ntt_c_context* ctx, const ntt_c_desc* d, const S
     const Plan* P = nullptr;
     st = ctx_plan(ctx, *d, 1ull << d->log_n, &P, forced);
     if (st != NTT_OK) return st;
-    // The four-step workspace is a context-owned per-stream buffer (no
-    // allocation once it is large enough).
-    void* scr = nullptr;
+    // The four-step workspace is the stream's reservation (never allocated here).
     const Store sd = (d->flags & NTT_FLAG_U64) ? Store::U64 : Store::U32;
-    st = ctx_scratch(ctx, (hipStream_t)stream, plan_scratch_elems(*P, sd) * vbytes(P->arith), &scr);
+    Workspace ws;
+    st = find_workspace(ctx->smu, ctx->work, (hipStream_t)stream, plan_scratch_elems(*P, sd) * vbytes(P->arith), &ws);
     if (st != NTT_OK) return st;
-    return run_plan(*P, d->data, sd, 1, scr, (hipStream_t)stream);
+    return run_plan(*P, d->data, sd, 1, ws.ptr, (hipStream_t)stream);
 }
 
 extern "C" ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* d, void* stream) {
@@ -1573,6 +1657,32 @@ extern "C" ntt_status ntt_c_event_destroy(void* e) {
 }
 extern "C" ntt_status ntt_c_event_record(void* e, void* s) {
     HIP_TRY(hipEventRecord((hipEvent_t)e, (hipStream_t)s));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_graph_begin(void* s) {
+    if (!s) return fail(NTT_ERR_INVALID_ARG, "graph_begin: capture needs a created stream, not NULL");
+    HIP_TRY(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeGlobal));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_graph_end(void* s, void** exec) {
+    if (!s || !exec) return fail(NTT_ERR_INVALID_ARG, "graph_end: null argument");
+    *exec = nullptr;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture((hipStream_t)s, &g));
+    hipGraphExec_t x = nullptr;
+    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(NTT_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    *exec = x;
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_graph_launch(void* exec, void* s) {
+    if (!exec) return fail(NTT_ERR_INVALID_ARG, "graph_launch: null graph");
+    HIP_TRY(hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)s));
+    return NTT_OK;
+}
+extern "C" ntt_status ntt_c_graph_destroy(void* exec) {
+    if (exec) HIP_TRY(hipGraphExecDestroy((hipGraphExec_t)exec));
     return NTT_OK;
 }
 extern "C" ntt_status ntt_c_event_elapsed_ms(float* ms, void* a, void* b) {

@@ -84,6 +84,10 @@ def lib():
         "ntt_c_rowmaj_from_pretransposed": [vp, vp, i32, i32, vp],
         "ntt_c_host": [P(u64), u64, P(ModularParams), i32],
         "ntt_c_create_params": [u64, u64, P(ModularParams)],
+        "ntt_c_config_reserve": [P(ConfigStruct), vp, u64],
+        "ntt_c_context_reserve": [vp, vp, u64],
+        "ntt_c_context_prepare": [vp, P(Desc), ctypes.c_size_t, vp],
+        "ntt_c_workspace_bytes": [P(Desc), P(u64)],
         "ntt_c_context_create": [P(vp)],
         "ntt_c_context_destroy": [vp],
         "ntt_c_transform": [vp, P(Desc), vp],
@@ -107,6 +111,10 @@ def lib():
         "ntt_c_event_destroy": [vp],
         "ntt_c_event_record": [vp, vp],
         "ntt_c_event_elapsed_ms": [P(ctypes.c_float), vp, vp],
+        "ntt_c_graph_begin": [vp],
+        "ntt_c_graph_end": [vp, P(vp)],
+        "ntt_c_graph_launch": [vp, vp],
+        "ntt_c_graph_destroy": [vp],
         "ntt_c_bconv_create": [P(u64), u32, P(u64), u32, P(vp)],
         "ntt_c_bconv": [vp, vp, vp, u64, vp],
         "ntt_c_bconv_destroy": [vp],
@@ -223,6 +231,28 @@ class Event:
             self.ptr = None
 
 
+class Graph:
+    """Calls captured on a stream (global capture mode) and replayed as a hipGraph."""
+
+    def __init__(self, stream, record):
+        check(lib().ntt_c_graph_begin(stream.ptr))
+        try:
+            record()
+        finally:
+            p = ctypes.c_void_p()
+            st = lib().ntt_c_graph_end(stream.ptr, ctypes.byref(p))
+        check(st)
+        self.ptr = p.value
+
+    def launch(self, stream):
+        check(lib().ntt_c_graph_launch(self.ptr, stream.ptr))
+
+    def destroy(self):
+        if self.ptr:
+            check(lib().ntt_c_graph_destroy(self.ptr))
+            self.ptr = None
+
+
 def device_count():
     c = ctypes.c_int()
     check(lib().ntt_c_device_count(ctypes.byref(c)))
@@ -255,6 +285,10 @@ class Config:
     def mu(self):
         return self.ptr.contents.mu
 
+    def reserve(self, stream, max_batch=1):
+        """Workspace of `stream` for calls of up to max_batch polynomials (set-up)."""
+        check(lib().ntt_c_config_reserve(self.ptr, _sp(stream), max_batch))
+
     def forward(self, buf, variant="optimized", stream=None):
         check(lib().ntt_c_forward(buf.ptr, self.ptr, VARIANT[variant], _sp(stream)))
 
@@ -286,6 +320,14 @@ class Config:
         if self.ptr:
             check(lib().ntt_c_cleanup(self.ptr))
             self.ptr = None
+
+
+def workspace_bytes(q, log_n, flags):
+    """Workspace bytes one transform of this kind needs (ntt_c_workspace_bytes)."""
+    d = Desc(1, q, log_n, flags)
+    out = ctypes.c_uint64()
+    check(lib().ntt_c_workspace_bytes(ctypes.byref(d), ctypes.byref(out)))
+    return out.value
 
 
 def create_params(q, n):
@@ -368,11 +410,28 @@ class Pmac:
             self.ptr = None
 
 
+DEFAULT_WORKSPACE = 256 << 20  # any single transform up to 2^25 x 8 bytes
+
+
 class Context:
-    def __init__(self):
+    def __init__(self, workspace_bytes=DEFAULT_WORKSPACE, stream=None):
+        """A plan/twiddle cache; reserves `workspace_bytes` of four-step
+        workspace for `stream` (the NULL stream by default) at once, since
+        transforms never allocate (0: reserve nothing)."""
         p = ctypes.c_void_p()
         check(lib().ntt_c_context_create(ctypes.byref(p)))
         self.ptr = p.value
+        if workspace_bytes:
+            self.reserve(stream, workspace_bytes)
+
+    def reserve(self, stream, nbytes):
+        check(lib().ntt_c_context_reserve(self.ptr, _sp(stream), nbytes))
+
+    def prepare(self, descs, stream=None):
+        """Builds the plans of (ptr, q, log_n, flags) descriptors and reserves
+        the workspace one of them needs on `stream`."""
+        arr = (Desc * len(descs))(*[Desc(*d) for d in descs])
+        check(lib().ntt_c_context_prepare(self.ptr, arr, len(descs), _sp(stream)))
 
     def table_bytes(self):
         return int(lib().ntt_c_context_table_bytes(self.ptr))

@@ -63,7 +63,7 @@ def test_init_rejects_bad_arguments_before_touching_the_device(nt):
 
 
 def test_descriptor_validation_is_host_side(nt):
-    ctx = nt.Context()
+    ctx = nt.Context(workspace_bytes=0)
     with pytest.raises(nt.NttError) as e:
         ctx.transform(0, BB, 10, 0)                                  # null data
     assert e.value.status == 1
