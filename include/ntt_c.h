@@ -89,6 +89,21 @@ ntt_status ntt_c_cleanup(ntt_c_config* cfg);
  * thread issues transforms on that stream. */
 ntt_status ntt_c_config_reserve(const ntt_c_config* cfg, void* stream, uint64_t max_batch);
 
+/* No reference counterpart; for callers that cannot allocate device memory
+ * at all (code under hipGraph capture, tight serving loops, hosts that forbid
+ * allocation).  The transform of ntt_c_forward / ntt_c_inverse /
+ * ntt_c_forward_negacyclic / ntt_c_inverse_negacyclic (flags NTT_FLAG_INVERSE
+ * | NTT_FLAG_NEGACYCLIC) on uint64 data in place, with its four-step
+ * workspace in a static 32 MiB device array of the library instead of a
+ * reserved buffer: no hipMalloc or hipFree, on any stream.  A transform needs
+ * at most 8 bytes of workspace per coefficient, so every n <= 2^22 fits;
+ * transforms whose workspace exceeds the 32 MiB (some plans at 2^23, all at
+ * 2^24 and above) return NTT_ERR_UNSUPPORTED and leave the data alone.  All calls on one GPU share
+ * that GPU's array: calls that overlap on one GPU (concurrent streams or host
+ * threads) race; calls ordered on one stream do not.  Nothing in the array
+ * needs to start at zero (every word is written before it is read). */
+ntt_status ntt_c_config_transform_noalloc(uint64_t* d_data, const ntt_c_config* cfg, uint32_t flags, void* stream);
+
 /* ntt_forward_{basic,four_step,tlmop,optimized} / ntt_inverse_*, reference
  * include/ntt.cuh:42-57 (src/ntt_basic.cu:268-334, src/ntt_four_step.cu:244-392,
  * src/ntt_optimized.cu:515-613).  In place, natural order in and out,

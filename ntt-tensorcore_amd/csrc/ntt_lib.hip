@@ -977,6 +977,7 @@ extern "C" ntt_status ntt_c_create_params(uint64_t q, uint64_t n, ntt_c_modular_
 }
 
 extern "C" ntt_status ntt_c_cleanup(ntt_c_config* cfg);
+static ntt_status static_scratch(void** out);
 
 // The config's tables and the NULL stream's workspace are made here; no
 // transform call allocates.  On failure everything built so far is freed.
@@ -1064,6 +1065,10 @@ extern "C" ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out) {
     I->arith = q < (1ull << 31) ? Arith::M32 : Arith::M64;
     C->impl = I;
     st = init_config(C, I);
+    if (st == NTT_OK) {
+        void* p = nullptr;
+        st = static_scratch(&p);  // the current device's copy of `scratch`, for the no-allocation entry point
+    }
     if (st != NTT_OK) {
         const std::string msg = g_last_error;
         (void)ntt_c_cleanup(C);
@@ -1081,6 +1086,61 @@ extern "C" ntt_status ntt_c_cleanup(ntt_c_config* cfg) {
     delete (ConfigImpl*)cfg->impl;
     delete cfg;
     return NTT_OK;
+}
+
+// ------------------------------------------------------------------ static scratch (no allocation at all)
+// Workspace of ntt_c_config_transform_noalloc: a file-scope device array, so
+// that entry point needs no hipMalloc, not even at set-up.  Each GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// `scratch` names a host-side stand-in, not device memory: the device address
+// of each GPU's copy comes from hipGetSymbolAddress, once per device, and
+// only that address is ever passed to a kernel.
+constexpr uint64_t kStaticScratchBytes = 32ull << 20;
+static __device__ __align__(256) uint32_t scratch[kStaticScratchBytes / 4];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_static_mu;
+static void* g_static_ptr[kMaxDevices];  // device address of `scratch` on each device, or null
+
+// The current device's address of `scratch` (looked up on the first call
+// for the device, with that device current).
+static ntt_status static_scratch(void** out) {
+    *out = nullptr;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(NTT_ERR_UNSUPPORTED, "static scratch: device index out of range");
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (!g_static_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(NTT_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_static_ptr[dev] = p;
+    }
+    *out = g_static_ptr[dev];
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_config_transform_noalloc(uint64_t* d, const ntt_c_config* cfg, uint32_t flags,
+                                                     void* stream) {
+    if (!d || !cfg || !cfg->impl) return fail(NTT_ERR_INVALID_ARG, "transform_noalloc: null data or config");
+    if (flags & ~(uint32_t)(NTT_FLAG_INVERSE | NTT_FLAG_NEGACYCLIC))
+        return fail(NTT_ERR_INVALID_ARG, "transform_noalloc: flags other than INVERSE | NEGACYCLIC");
+    ConfigImpl* I = (ConfigImpl*)cfg->impl;
+    const int k = ((flags & NTT_FLAG_NEGACYCLIC) ? 2 : 0) + ((flags & NTT_FLAG_INVERSE) ? 1 : 0);
+    if (!I->plans[k]) return fail(I->plan_status[k], I->plan_error[k]);
+    const Plan& P = *I->plans[k];
+    const uint64_t need = plan_scratch_elems(P, Store::U64) * vbytes(P.arith);
+    if (need > kStaticScratchBytes)
+        return fail(NTT_ERR_UNSUPPORTED, "transform_noalloc: this transform needs " + std::to_string(need) +
+                                             " bytes of scratch, more than the " +
+                                             std::to_string(kStaticScratchBytes) + " of the static array");
+    void* ptr = nullptr;
+    if (need) {
+        ntt_status st = static_scratch(&ptr);
+        if (st != NTT_OK) return st;
+    }
+    return run_plan(P, d, Store::U64, 1, ptr, (hipStream_t)stream);
 }
 
 extern "C" ntt_status ntt_c_config_reserve(const ntt_c_config* cfg, void* stream, uint64_t max_batch) {

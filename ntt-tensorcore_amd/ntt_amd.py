@@ -85,6 +85,7 @@ def lib():
         "ntt_c_host": [P(u64), u64, P(ModularParams), i32],
         "ntt_c_create_params": [u64, u64, P(ModularParams)],
         "ntt_c_config_reserve": [P(ConfigStruct), vp, u64],
+        "ntt_c_config_transform_noalloc": [vp, P(ConfigStruct), u32, vp],
         "ntt_c_context_reserve": [vp, vp, u64],
         "ntt_c_context_prepare": [vp, P(Desc), ctypes.c_size_t, vp],
         "ntt_c_workspace_bytes": [P(Desc), P(u64)],
