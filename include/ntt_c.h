@@ -201,6 +201,10 @@ ntt_status ntt_c_workspace_bytes(const ntt_c_desc* desc, uint64_t* bytes);
 /* One transform, one polynomial, launched on its own (the per-polynomial
  * path); its workspace is the stream's reservation (NTT_ERR_OOM if short). */
 ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* desc, void* stream);
+/* ntt_c_transform on each descriptor in turn, on one stream: the reference's
+ * one-polynomial-per-launch batch loop (src/ntt_basic.cu:337-362) issued from
+ * native code, the comparison point of the descriptor batch. */
+ntt_status ntt_c_transform_each(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count, void* stream);
 
 /* One transform with an explicit radix split n = 2^log_r[0] * ... * 2^log_r[npass-1]
  * (npass <= 4), a tile of 2^log_e elements per workgroup, and bit i of

@@ -1639,6 +1639,18 @@ extern "C" ntt_status ntt_c_transform(ntt_c_context* ctx, const ntt_c_desc* d, v
     return transform_one(ctx, d, nullptr, stream);
 }
 
+// One transform call per descriptor, in order, on one stream: the
+// reference's per-polynomial batch loop (src/ntt_basic.cu:337-362) issued from
+// native code, each polynomial its own launch chain.
+extern "C" ntt_status ntt_c_transform_each(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count, void* stream) {
+    if (!ctx || (count && !descs)) return fail(NTT_ERR_INVALID_ARG, "transform_each: null argument");
+    for (size_t i = 0; i < count; i++) {
+        ntt_status st = transform_one(ctx, &descs[i], nullptr, stream);
+        if (st != NTT_OK) return st;
+    }
+    return NTT_OK;
+}
+
 extern "C" ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc* d, uint32_t npass,
                                             const uint32_t* log_r, uint32_t log_e, uint32_t mfma_mask, void* stream) {
     if (!log_r || npass < 1 || npass > 4) return fail(NTT_ERR_INVALID_ARG, "transform_split: bad split");

@@ -93,6 +93,7 @@ def lib():
         "ntt_c_context_destroy": [vp],
         "ntt_c_transform": [vp, P(Desc), vp],
         "ntt_c_transform_split": [vp, P(Desc), u32, P(u32), u32, u32, vp],
+        "ntt_c_transform_each": [vp, P(Desc), ctypes.c_size_t, vp],
         "ntt_c_batch_prepare": [vp, P(Desc), ctypes.c_size_t, P(vp)],
         "ntt_c_batch_launch": [vp, vp],
         "ntt_c_batch_launch_graph": [vp, vp],
@@ -440,6 +441,15 @@ class Context:
     def transform(self, data_ptr, q, log_n, flags, stream=None):
         d = Desc(data_ptr, q, log_n, flags)
         check(lib().ntt_c_transform(self.ptr, ctypes.byref(d), _sp(stream)))
+
+    def transform_each(self, descs, stream=None):
+        """One ntt_c_transform per (ptr, q, log_n, flags), in order, from native code."""
+        arr = descs if isinstance(descs, ctypes.Array) else (Desc * len(descs))(*[Desc(*d) for d in descs])
+        check(lib().ntt_c_transform_each(self.ptr, arr, len(arr), _sp(stream)))
+
+    @staticmethod
+    def desc_array(descs):
+        return (Desc * len(descs))(*[Desc(*d) for d in descs])
 
     def transform_split(self, data_ptr, q, log_n, flags, log_r, log_e, stream=None, mfma=0):
         d = Desc(data_ptr, q, log_n, flags)

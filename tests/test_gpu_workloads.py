@@ -118,6 +118,12 @@ def test_uneven_512_one_chain_per_polynomial(nt, uneven, shipped_table):
         nt.device_sync()
         for i, (j, o, e) in enumerate(zip(jobs, offs, expect)):
             assert np.array_equal(buf.download(np.uint32, 1 << j["log_n"], o).astype(np.uint64), e), (i, j)
+        # the same loop issued from native code (ntt_c_transform_each)
+        _upload(buf, polys, offs, np.uint32)
+        ctx.transform_each([(buf.ptr + o, j["q"], j["log_n"], _flags(nt, j)) for j, o in zip(jobs, offs)])
+        nt.device_sync()
+        for i, (j, o, e) in enumerate(zip(jobs, offs, expect)):
+            assert np.array_equal(buf.download(np.uint32, 1 << j["log_n"], o).astype(np.uint64), e), (i, j)
     finally:
         ctx.destroy()
         buf.free()
