@@ -217,6 +217,18 @@ __device__ __forceinline__ T gld(const T* p) {
 #endif
 }
 
+// Store through a global-address-space pointer (the coefficient buffers are
+// device memory): a generic store would be a flat store, which also counts in
+// lgkmcnt, so every LDS wait (lds_barrier) would wait for it too.
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    *(__attribute__((address_space(1))) T*)p = v;
+#else
+    *p = v;
+#endif
+}
+
 // A table of Shoup pairs staged in LDS at byte offset `off` (a multiple of the
 // pair's size): indexes and offsets like the global pointer it replaces.  A
 // pair moves as one aligned 8- or 16-byte access.
@@ -386,24 +398,24 @@ template <class S>
 __device__ __forceinline__ void load4(const S* p, S v[4]);
 template <>
 __device__ __forceinline__ void load4<uint32_t>(const uint32_t* p, uint32_t v[4]) {
-    const uint4 t = *reinterpret_cast<const uint4*>(p);
+    const uint4 t = gld(reinterpret_cast<const uint4*>(p));
     v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
 }
 template <>
 __device__ __forceinline__ void load4<uint64_t>(const uint64_t* p, uint64_t v[4]) {
-    const ulonglong2 a = reinterpret_cast<const ulonglong2*>(p)[0], b = reinterpret_cast<const ulonglong2*>(p)[1];
+    const ulonglong2 a = gld(reinterpret_cast<const ulonglong2*>(p)), b = gld(reinterpret_cast<const ulonglong2*>(p) + 1);
     v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
 }
 template <class S>
 __device__ __forceinline__ void store4(S* p, const S v[4]);
 template <>
 __device__ __forceinline__ void store4<uint32_t>(uint32_t* p, const uint32_t v[4]) {
-    *reinterpret_cast<uint4*>(p) = make_uint4(v[0], v[1], v[2], v[3]);
+    gst(reinterpret_cast<uint4*>(p), make_uint4(v[0], v[1], v[2], v[3]));
 }
 template <>
 __device__ __forceinline__ void store4<uint64_t>(uint64_t* p, const uint64_t v[4]) {
-    reinterpret_cast<ulonglong2*>(p)[0] = make_ulonglong2(v[0], v[1]);
-    reinterpret_cast<ulonglong2*>(p)[1] = make_ulonglong2(v[2], v[3]);
+    gst(reinterpret_cast<ulonglong2*>(p), make_ulonglong2(v[0], v[1]));
+    gst(reinterpret_cast<ulonglong2*>(p) + 1, make_ulonglong2(v[2], v[3]));
 }
 
 // Vector loads (or stores) per thread kept in flight: 8 of 32-bit words.  The
@@ -532,7 +544,7 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint3
                     Sout v[4] = {(Sout)x[0], (Sout)x[PER > 1 ? 1 : 0], (Sout)x[PER > 2 ? 2 : 0], (Sout)x[PER > 3 ? 3 : 0]};
                     store4<Sout>(p, v);
                 } else {
-                    *p = (Sout)x[0];
+                    gst(p, (Sout)x[0]);
                 }
             }
         }
@@ -562,7 +574,7 @@ __device__ __forceinline__ void store_loop_step(const PassJob& J, const Mod32& m
     const uint32_t kstep = 4u << log_tpc;
     const uint32_t q = mod.q, qinv = inv_mod_2_32(q);
     const TwP<V>* ct = reinterpret_cast<const TwP<V>*>(J.tw_col) + (size_t)cg * 4u;
-    const TwP<V> B1 = ct[0], B2 = ct[1], B3 = ct[2], Wb = ct[3];
+    const TwP<V> B1 = gld(ct), B2 = gld(ct + 1), B3 = gld(ct + 2), Wb = gld(ct + 3);
     // G = base^(e_d*c + s*4m) * mult * 2^32 mod q
     const uint32_t b0 = J.lo_bits, b1 = J.mid_bits, b01 = b0 + b1;
     const uint32_t e0 = (J.e_d * cg + (J.e_a * cg + J.e_b) * (4u * m)) & J.e_mask;
@@ -680,7 +692,7 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
                 for (int e = 0; e < PER; e++) v[e] = 0;
             }
         } else {
-            v[0] = ok ? *p : (Sin)0;
+            v[0] = ok ? gld(p) : (Sin)0;
         }
     };
     const LoadMul<V> mul(MUL ? J.q : 3u, J.mul_r2);

@@ -624,6 +624,11 @@ static uint32_t job_lds(const PassJob& J, Arith ar) {
 
 // Launch kind of a job: which kernel instantiation runs it.
 static uint32_t job_kind(const PassJob& J) { return J.flags & (JF_MFMA | JF_POST4); }
+// Jobs of one batch launch share their round kind (matrix-core or VALU
+// rounds: different instantiations), while jobs with and without a four-step
+// post-multiply share one launch of the post-multiply instantiation, which
+// runs every post kind (store_dispatch picks it per job).
+static uint32_t launch_key(const PassJob& J) { return J.flags & JF_MFMA; }
 
 // log2 of every stride of a job.  make_jobs only makes strides that are powers
 // of two (N >> lr, 1, N, 2^ll, 2^(lr+ll)) or zero where the stride only
@@ -1484,9 +1489,9 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
     // Group by (phase, arith, storage); within a group order by LDS size so
     // that big tiles start first.
     std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
-        return std::make_tuple(a.phase, (int)a.ar, (int)a.sin, (int)a.sout, job_kind(a.job),
+        return std::make_tuple(a.phase, (int)a.ar, (int)a.sin, (int)a.sout, launch_key(a.job),
                                -(int)(a.job.log_r + a.job.log_c)) <
-               std::make_tuple(b.phase, (int)b.ar, (int)b.sin, (int)b.sout, job_kind(b.job),
+               std::make_tuple(b.phase, (int)b.ar, (int)b.sin, (int)b.sout, launch_key(b.job),
                                -(int)(b.job.log_r + b.job.log_c));
     });
     for (size_t i = 0; i < items.size();) {
@@ -1496,12 +1501,13 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
         memset(&g.inl, 0, sizeof(g.inl));
         g.sin = items[i].sin;
         g.sout = items[i].sout;
-        g.kind = job_kind(items[i].job);
+        g.kind = 0;
         std::vector<PassJob> jobs;
         uint32_t blocks = 0, lds = 0;
         while (j < items.size() && items[j].phase == items[i].phase && items[j].ar == g.ar && items[j].sin == g.sin &&
-               items[j].sout == g.sout && job_kind(items[j].job) == g.kind) {
+               items[j].sout == g.sout && launch_key(items[j].job) == launch_key(items[i].job)) {
             PassJob J = items[j].job;
+            g.kind |= job_kind(J);
             J.tile_begin = blocks;
             blocks += job_tiles(J);
             lds = std::max(lds, job_lds(J, g.ar));
