@@ -38,6 +38,8 @@ struct TwP {  // Shoup pair: w and floor(w * 2^bits(V) / q)
 // q < 2^31: every intermediate fits 32 bits (2q < 2^32).
 struct Mod32 {
     using V = uint32_t;
+    static constexpr bool kLazy = false;  // butterflies keep canonical values (4q does not fit 32 bits)
+    static constexpr int kMinWaves = 2;   // waves per SIMD a 512-thread workgroup needs anyway
     uint32_t q;
     __device__ __forceinline__ V reduce2(V r) const { return min(r, r - q); }  // [0,2q) -> [0,q)
     __device__ __forceinline__ V mul(V x, TwP<V> t) const { return reduce2(mul_lazy(x, t)); }
@@ -108,14 +110,32 @@ struct LoadMul<uint64_t> {
 };
 
 // q < 2^62: 2q < 2^63, Shoup with a 64-bit quotient.
+// The VALU rounds run Harvey's lazy butterflies (kLazy): values between
+// levels are any representatives below 4q (4q < 2^64), so a butterfly is one
+// lazy Shoup product (any input below 2^64 gives [0, 2q)), one conditional
+// subtraction of 2q on the other input, an add and a subtract:
+//   a' = a mod 2q in [0, 2q),  t = b w mod q in [0, 2q),
+//   (a' + t, a' - t + 2q), both in [0, 4q).
+// The store canonicalises (canon4), or the post-multiply does (mul takes any
+// input below 2^64 and returns [0, q)).
 struct Mod64 {
     using V = uint64_t;
+    static constexpr bool kLazy = true;
+    // 4 waves per SIMD (at most 128 VGPRs): the 64-bit rounds otherwise take
+    // a few registers more and drop to 3 waves, which costs far more than the
+    // lazy butterflies save.
+    static constexpr int kMinWaves = 4;
     uint64_t q;
     __device__ __forceinline__ V reduce2(V r) const { return r >= q ? r - q : r; }
     __device__ __forceinline__ V mul(V x, TwP<V> t) const { return reduce2(mul_lazy(x, t)); }
     __device__ __forceinline__ V mul_lazy(V x, TwP<V> t) const { return x * t.w - __umul64hi(x, t.wp) * q; }
     __device__ __forceinline__ V add(V a, V b) const { return reduce2(a + b); }
     __device__ __forceinline__ V sub(V a, V b) const { return reduce2(a - b + q); }
+    __device__ __forceinline__ V half4(V a) const {  // [0, 4q) -> [0, 2q)
+        const V q2 = q << 1;
+        return a >= q2 ? a - q2 : a;
+    }
+    __device__ __forceinline__ V canon4(V a) const { return reduce2(half4(a)); }  // [0, 4q) -> [0, q)
 };
 
 enum PostMode : uint32_t { POST_NONE = 0, POST_FOURSTEP = 2, POST_SCALE = 3 };
@@ -309,10 +329,17 @@ __device__ __forceinline__ void ct_unit(const ZP* zb, const M& mod, uint32_t slo
 #pragma unroll
             for (int v0 = 0; v0 < half; v0++) {
                 const int v = g * 2 * half + v0;
-                const V a = x[v];
-                const V b = (ONE0 && g == 0) ? x[v + half] : mod.mul(x[v + half], wt);
-                x[v] = mod.add(a, b);
-                x[v + half] = mod.sub(a, b);
+                if constexpr (M::kLazy) {
+                    const V a = mod.half4(x[v]);
+                    const V t = (ONE0 && g == 0) ? mod.half4(x[v + half]) : mod.mul_lazy(x[v + half], wt);
+                    x[v] = a + t;
+                    x[v + half] = a - t + (mod.q << 1);
+                } else {
+                    const V a = x[v];
+                    const V b = (ONE0 && g == 0) ? x[v + half] : mod.mul(x[v + half], wt);
+                    x[v] = mod.add(a, b);
+                    x[v + half] = mod.sub(a, b);
+                }
             }
         }
     }
@@ -537,6 +564,9 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint3
             } else if (POST == 1) {
 #pragma unroll
                 for (int e = 0; e < PER; e++) x[e] = mod.mul(x[e], scale);
+            } else if constexpr (M::kLazy) {  // lazy butterfly outputs below 4q
+#pragma unroll
+                for (int e = 0; e < PER; e++) x[e] = mod.canon4(x[e]);
             }
             if (!PRED || (lane_ok && cg < ncols)) {
                 Sout* __restrict__ p = out + (ooff(uk, uc) + l_off);
@@ -903,7 +933,7 @@ __device__ __forceinline__ uint32_t find_job(const PassJob* __restrict__ jobs, u
 // kind.  (Forcing 8 waves per SIMD on the matrix-core shapes, 4 workgroups
 // per CU, fits without scratch but measured ~2% slower at 2^24.)
 template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC, bool MF>
-__global__ void __launch_bounds__(NT) pass_kernel(const PassJob* __restrict__ jobs, uint32_t njobs, PassJob inl) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(M::kMinWaves))) pass_kernel(const PassJob* __restrict__ jobs, uint32_t njobs, PassJob inl) {
     // The dynamic LDS (no static LDS: it starts at address 0) is addressed by
     // byte offset through lds_ld / lds_st.
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
