@@ -439,24 +439,50 @@ def extra_u64_abi(args, stream):
 def extra_klimb(stream):
     """K-limb big-prime path (include/ntt_arbitrary.cuh): forward + inverse over
     Montgomery-form limbs, n = 2^20 for the 48-bit (K=2) and Goldilocks (K=3)
-    fields, n = 2^16 for BN254 (K=11).  Algorithmic bytes: 2 n 4K per transform."""
+    fields, n = 2^16 for BN254 (K=11).  Algorithmic bytes: 2 n 4K per transform.
+    K = 2 and 3 run on the four-step pass kernels (ntt_c_arb_plan_*), timed
+    beside the limb kernels (ntt_c_arb_forward / inverse) whose output they
+    must equal; K = 11 has only the limb kernels."""
     import ntt_arb as A
     out = {}
     for K, log_n in ((2, 20), (3, 20), (11, 16)):
         n = 1 << log_n
         f = A.Field(A.FIELDS[K][0], K)
-        plan = A.ArbNtt(f, log_n)
+        limb = A.ArbNtt(f, log_n)
+        fast = A.ArbPlan(limb) if K <= 3 else None
         rng = np.random.default_rng(K)
         x = rng.integers(0, 1 << 24, size=(n, K), dtype=np.uint32)
         x[:, -1] = rng.integers(0, int(f.p_limbs[-1]), size=n, dtype=np.uint32)
         buf = nt.DeviceBuffer.from_array(x)
-        ms = timed(lambda: (plan.forward(buf, stream), plan.inverse(buf, stream)), 10, 2, stream)
-        stream.sync()
-        ok = np.array_equal(buf.download(np.uint32, n * K).reshape(n, K), x)
+
+        def run_of(t):
+            return lambda: (t.forward(buf, stream), t.inverse(buf, stream))
+        rec = {}
+        for name, t in (("limb_kernels", limb), ("plan", fast)):
+            if t is None:
+                continue
+            ms = timed(run_of(t), 10, 2, stream)
+            stream.sync()
+            _require(np.array_equal(buf.download(np.uint32, n * K).reshape(n, K), x), f"klimb K={K} {name} round trip")
+            rec[name] = ms / 10 / 2 / 1e3
+        if fast is not None:  # the plan's forward equals the limb kernels' forward, limb for limb
+            other = nt.DeviceBuffer.from_array(x)
+            fast.forward(buf, stream)
+            limb.forward(other, stream)
+            stream.sync()
+            _require(np.array_equal(buf.download(np.uint32, n * K), other.download(np.uint32, n * K)),
+                     f"klimb K={K} plan forward != limb kernels")
+            other.free()
+            fast.destroy()
         buf.free()
-        t = ms / 10 / 2 / 1e3
-        out[f"K{K}_2^{log_n}"] = {"us": t * 1e6, "Gbfly_s": W.butterflies(log_n) / t / 1e9,
-                                  "GB_s": 2 * n * 4 * K / t / 1e9, "roundtrip_ok": bool(ok)}
+        t = rec.get("plan", rec["limb_kernels"])
+        r = {"us": t * 1e6, "Gbfly_s": W.butterflies(log_n) / t / 1e9, "GB_s": 2 * n * 4 * K / t / 1e9,
+             "kernels": "four-step pass kernels (plan)" if "plan" in rec else "limb kernels",
+             "checked": "round trip" + (" + plan forward == limb kernels' forward" if "plan" in rec else "")}
+        if "plan" in rec:
+            r["limb_kernels_us"] = rec["limb_kernels"] * 1e6
+            r["speedup_vs_limb_kernels"] = rec["limb_kernels"] / rec["plan"]
+        out[f"K{K}_2^{log_n}"] = r
     return out
 
 

@@ -18,6 +18,11 @@
 //  * NTTBuffers keeps d_tfm8 / d_hada64 for source compatibility but leaves
 //    them null: the library's kernels need no per-size MMA tables, and
 //    run_forward_ct_tcu accepts and ignores them.
+//  * setup_ntt also builds a plan (ntt_c_arb_plan_create) when p fits 64-bit
+//    arithmetic (K <= 3: the K = 2 prime, Goldilocks), and run_forward /
+//    run_inverse, run_for_K and the negacyclic helpers run it on the
+//    four-step pass kernels; run_forward_ct / run_forward_ct_tcu keep the
+//    limb kernels.  Both give the same limbs.
 //  * Failed library calls print ntt_c_last_error() and abort() unless
 //    NTT_NO_ABORT is defined (the reference does no error checking here).
 #pragma once
@@ -436,6 +441,7 @@ struct NTTBuffers {
     BigInt<K> psi{};                   // primitive 2n-th root (zero if 2n exceeds the field's 2-adic range)
     DScalar<K>* d_psi = nullptr;       // psi^j R mod p
     DScalar<K>* d_psi_inv = nullptr;   // psi^-j R mod p
+    ntt_c_arb_plan* plan = nullptr;    // four-step pass kernels, when p fits 64-bit arithmetic
 };
 
 template <int K>
@@ -501,6 +507,11 @@ inline NTTBuffers<K> setup_ntt(int log_n, uint64_t seed = 0xC0FFEE) {
         upload_mont_<K>(B.d_psi, fw, B);
         upload_mont_<K>(B.d_psi_inv, iv, B);
     }
+    if (K <= 3) {  // moduli that fit 64-bit arithmetic run on the four-step pass kernels
+        const DScalar<K> root = to_mont_d<K>(w, B);
+        const ntt_status st = ntt_c_arb_plan_create(&B.plan, B.prime_d.limbs, B.np, K, (uint32_t)log_n, root.limbs);
+        if (st != NTT_ERR_UNSUPPORTED) check_(st, "setup_ntt (plan)");
+    }
     return B;
 }
 
@@ -510,6 +521,23 @@ inline void teardown_ntt(NTTBuffers<K>& B) {
         if (*p) hip_check_(hipFree(*p), "hipFree");
         *p = nullptr;
     }
+    if (B.plan) check_(ntt_c_arb_plan_destroy(B.plan), "teardown_ntt (plan)");
+    B.plan = nullptr;
+}
+
+// Forward / inverse transform of B's size on d, limb for limb the results of
+// run_forward_ct / run_inverse_ct with B's table: the four-step pass kernels
+// when B holds a plan (p fits 64-bit arithmetic), else the limb kernels.
+template <int K>
+inline void run_forward(NTTBuffers<K>& B, DScalar<K>* d, hipStream_t stream = 0) {
+    if (B.plan) check_(ntt_c_arb_plan_forward(B.plan, d->limbs, stream), "run_forward");
+    else run_forward_ct<K>(d, B.d_tw, (uint32_t)B.n, B.log_n, B.prime_d, B.np, stream);
+}
+
+template <int K>
+inline void run_inverse(NTTBuffers<K>& B, DScalar<K>* d, hipStream_t stream = 0) {
+    if (B.plan) check_(ntt_c_arb_plan_inverse(B.plan, d->limbs, stream), "run_inverse");
+    else run_inverse_ct<K>(d, B.d_tw, (uint32_t)B.n, B.log_n, B.prime_d, B.np, B.n_inv_mont, stream);
 }
 
 // Negacyclic transforms (no reference counterpart): A_k = sum_j a_j psi^(j(2k+1)),
@@ -518,13 +546,13 @@ template <int K>
 inline void run_forward_negacyclic(NTTBuffers<K>& B, DScalar<K>* d, hipStream_t stream = 0) {
     if (!B.d_psi) check_(NTT_ERR_UNSUPPORTED, "run_forward_negacyclic (no 2n-th root in this field)");
     pointwise_mul_mont<K>(d, B.d_psi, (uint32_t)B.n, B.prime_d, B.np, stream);
-    run_forward_ct<K>(d, B.d_tw, (uint32_t)B.n, B.log_n, B.prime_d, B.np, stream);
+    run_forward<K>(B, d, stream);
 }
 
 template <int K>
 inline void run_inverse_negacyclic(NTTBuffers<K>& B, DScalar<K>* d, hipStream_t stream = 0) {
     if (!B.d_psi_inv) check_(NTT_ERR_UNSUPPORTED, "run_inverse_negacyclic (no 2n-th root in this field)");
-    run_inverse_ct<K>(d, B.d_tw, (uint32_t)B.n, B.log_n, B.prime_d, B.np, B.n_inv_mont, stream);
+    run_inverse<K>(B, d, stream);
     pointwise_mul_mont<K>(d, B.d_psi_inv, (uint32_t)B.n, B.prime_d, B.np, stream);
 }
 
@@ -540,7 +568,7 @@ inline int run_for_K(int log_n, bool do_host_check = true) {
     }
     NTTBuffers<K> B = setup_ntt<K>(log_n);
     std::cout << "K=" << K << " (LIMB_BITS=" << LIMB_BITS << "), N=2^" << log_n << "=" << B.n << "\n";
-    auto run = [&] { run_forward_ct<K>(B.d_data, B.d_tw, (uint32_t)B.n, log_n, B.prime_d, B.np); };
+    auto run = [&] { run_forward<K>(B, B.d_data); };
     for (int it = 0; it < 3; it++) run();
     hip_check_(hipDeviceSynchronize(), "sync");
     hipEvent_t s, e;

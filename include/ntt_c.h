@@ -75,8 +75,9 @@ enum {
 /* ntt_init, reference src/ntt_basic.cu:217-249 (include/ntt.cuh:35).  Builds
  * params by the reference's generator rule, the device twiddle tables, the
  * transform plans and the NULL stream's workspace (one call of any kind; a
- * batch runs in chunks that fit it).  q < 2^62 prime with n | q-1.  On
- * failure nothing stays allocated. */
+ * batch runs in chunks that fit it).  q prime with n | q-1, either q < 2^62
+ * or the Goldilocks prime 2^64 - 2^32 + 1 (its own reduction; the reference's
+ * uint64 sums overflow there).  On failure nothing stays allocated. */
 ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out);
 /* ntt_cleanup, reference src/ntt_basic.cu:251-257 (include/ntt.cuh:38). */
 ntt_status ntt_c_cleanup(ntt_c_config* cfg);
@@ -174,7 +175,7 @@ uint64_t ntt_c_find_primitive_root(uint64_t q, uint64_t n);
 
 typedef struct {
     void* data;       /* device pointer, n coefficients (uint32 or uint64 per NTT_FLAG_U64), in place */
-    uint64_t q;       /* prime, q < 2^31 with uint32 storage; q < 2^62 with NTT_FLAG_U64 */
+    uint64_t q;       /* prime, q < 2^31 with uint32 storage; q < 2^62 or 2^64 - 2^32 + 1 with NTT_FLAG_U64 */
     uint32_t log_n;   /* 1 .. 26 */
     uint32_t flags;   /* NTT_FLAG_* */
 } ntt_c_desc;
@@ -316,6 +317,24 @@ ntt_status ntt_c_arb_pointwise_mul(uint32_t* d_a, const uint32_t* d_b, uint32_t 
  * run_for_K, reference :2731-2746, 2884-2888). */
 ntt_status ntt_c_arb_scale(uint32_t* d_a, uint32_t n, const uint32_t* scalar_limbs, const uint32_t* prime_limbs,
                            uint32_t np, uint32_t K, void* stream);
+
+/* K-limb transforms on the four-step pass kernels, for moduli that fit 64-bit
+ * arithmetic: K = 1 or 2, or K = 3 with p < 2^62 or p = 2^64 - 2^32 + 1
+ * (Goldilocks, FieldConfig<3>).  Same data and the same results, limb for limb,
+ * as ntt_c_arb_forward / ntt_c_arb_inverse with d_tw[k] = w^k R mod p, where
+ * root_mont = w R mod p (d_tw[1]; K limbs, host memory): the transform is
+ * linear, so it runs on the Montgomery-form values as plain residues.  Other
+ * moduli return NTT_ERR_UNSUPPORTED (use ntt_c_arb_forward).  The plan owns
+ * its tables and workspace (allocated here, at set-up); calls with one plan
+ * must not overlap (one stream, or synchronised).  Replaces the reference's
+ * run_forward_ct / run_forward_ct_tcu (include/ntt_arbitrary.cuh:2547-2646)
+ * for these fields; the inverse has no reference counterpart. */
+typedef struct ntt_c_arb_plan ntt_c_arb_plan;
+ntt_status ntt_c_arb_plan_create(ntt_c_arb_plan** plan, const uint32_t* prime_limbs, uint32_t np, uint32_t K,
+                                 uint32_t log_n, const uint32_t* root_mont);
+ntt_status ntt_c_arb_plan_forward(const ntt_c_arb_plan* plan, uint32_t* d_data, void* stream);
+ntt_status ntt_c_arb_plan_inverse(const ntt_c_arb_plan* plan, uint32_t* d_data, void* stream);
+ntt_status ntt_c_arb_plan_destroy(ntt_c_arb_plan* plan);
 
 /* ---------------- RNS kernels around the transform.  No reference
  * counterpart: the reference names them as absent (src/ntt_optimized.cu:510-513,

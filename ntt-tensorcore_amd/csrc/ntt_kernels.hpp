@@ -83,11 +83,15 @@ __device__ __forceinline__ uint64_t mont64(uint64_t x, uint64_t y, uint64_t q, u
 }
 
 // a*b mod q of canonical residues for the product fused into a pass's load
-// (JF_MUL_IN): a Montgomery product, then one by R^2 mod q to remove R^-1.
-template <class V>
+// (JF_MUL_IN), by arithmetic: a Montgomery product, then one by R^2 mod q to
+// remove R^-1 (Mod32, Mod64); the field's own product (ModGold).
+template <class M>
 struct LoadMul;
+struct Mod32;
+struct Mod64;
+struct ModGold;
 template <>
-struct LoadMul<uint32_t> {
+struct LoadMul<Mod32> {
     uint32_t q, qinv, r2;
     __device__ __forceinline__ explicit LoadMul(uint64_t q_, uint64_t r2_)
         : q((uint32_t)q_), qinv(inv_mod_2_32((uint32_t)q_)), r2((uint32_t)r2_) {}
@@ -96,7 +100,7 @@ struct LoadMul<uint32_t> {
     }
 };
 template <>
-struct LoadMul<uint64_t> {
+struct LoadMul<Mod64> {
     uint64_t q, qneg_inv, r2;
     __device__ __forceinline__ explicit LoadMul(uint64_t q_, uint64_t r2_) : q(q_), qneg_inv(0), r2(r2_) {
         uint64_t x = q_;  // Newton from q (right to 3 bits): 5 steps reach 96 bits
@@ -136,6 +140,58 @@ struct Mod64 {
         return a >= q2 ? a - q2 : a;
     }
     __device__ __forceinline__ V canon4(V a) const { return reduce2(half4(a)); }  // [0, 4q) -> [0, q)
+};
+
+// The Goldilocks field, q = 2^64 - 2^32 + 1.  2^64 = 2^32 - 1 and 2^96 = -1
+// mod q, so the 128-bit product h*2^64 + l (h = h1*2^32 + h0) is congruent to
+// l - h1 + h0*(2^32 - 1): a subtraction and an addition of 64-bit words, each
+// corrected by 2^32 - 1 when it borrows or carries (an exact wrap of 2^64 is
+// one of 2^32 - 1 mod q), give a representative below 2^64.  No Shoup
+// quotient is needed (the tables' second word is ignored), and values stay
+// canonical between levels (4q does not fit 64 bits).
+constexpr uint64_t kGoldilocks = 0xFFFFFFFF00000001ull;
+__device__ __forceinline__ uint64_t gold_reduce128(uint64_t lo, uint64_t hi) {
+    constexpr uint64_t eps = 0xFFFFFFFFull;  // 2^64 mod q
+    const uint64_t h1 = hi >> 32, h0 = hi & eps;
+    uint64_t t = lo - h1;
+    if (lo < h1) t -= eps;  // t wrapped by 2^64 (t >= 2^64 - 2^32, so no second wrap)
+    const uint64_t u = (h0 << 32) - h0;  // h0 * (2^32 - 1) < 2^64
+    uint64_t r = t + u;
+    if (r < u) r += eps;  // carried: r < u < 2^64 - 2^33, so no second carry
+    return r;
+}
+__device__ __forceinline__ uint64_t gold_mul_any(uint64_t x, uint64_t y) {  // [0, 2^64), any x, y
+    return gold_reduce128(x * y, __umul64hi(x, y));
+}
+struct ModGold {
+    using V = uint64_t;
+    static constexpr bool kLazy = false;
+    static constexpr int kMinWaves = 4;  // as Mod64: at most 128 VGPRs
+    uint64_t q;                          // kGoldilocks (the job's modulus)
+    __device__ __forceinline__ V reduce2(V r) const { return r >= kGoldilocks ? r - kGoldilocks : r; }  // [0, 2^64) -> [0, q)
+    __device__ __forceinline__ V mul_lazy(V x, TwP<V> t) const { return gold_mul_any(x, t.w); }
+    __device__ __forceinline__ V mul(V x, TwP<V> t) const { return reduce2(gold_mul_any(x, t.w)); }
+    // a + b < 2q: a carry out of 64 bits leaves a + b - 2^64 + (2^32 - 1) = a + b - q < q
+    __device__ __forceinline__ V add(V a, V b) const {
+        const V s = a + b;
+        return s < a ? s + 0xFFFFFFFFull : reduce2(s);
+    }
+    // a < b: a - b + 2^64 - (2^32 - 1) = a - b + q
+    __device__ __forceinline__ V sub(V a, V b) const {
+        const V d = a - b;
+        return a < b ? d - 0xFFFFFFFFull : d;
+    }
+    __device__ __forceinline__ V half4(V a) const { return reduce2(a); }
+    __device__ __forceinline__ V canon4(V a) const { return reduce2(a); }
+};
+
+template <>
+struct LoadMul<ModGold> {
+    __device__ __forceinline__ explicit LoadMul(uint64_t, uint64_t) {}
+    __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const {
+        const uint64_t r = gold_mul_any(a, b);
+        return r >= kGoldilocks ? r - kGoldilocks : r;
+    }
 };
 
 enum PostMode : uint32_t { POST_NONE = 0, POST_FOURSTEP = 2, POST_SCALE = 3 };
@@ -685,8 +741,9 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
 // stores in[i]*in2[i] mod q (JF_MUL_IN).
 //   pre() runs once, after the first batch's global loads are issued and before
 //   their LDS stores, so that loads it issues share their latency.
-template <int PER, bool PRED, int LR, int LC, class Sin, class V, bool MUL = false, class Pre>
+template <int PER, bool PRED, int LR, int LC, class Sin, class M, bool MUL = false, class Pre>
 __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint32_t col0, Pre pre) {
+    using V = typename M::V;
     constexpr uint32_t VB = sizeof(V);
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
     const uint32_t R = 1u << log_r, C = 1u << log_c;
@@ -725,7 +782,7 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
             v[0] = ok ? gld(p) : (Sin)0;
         }
     };
-    const LoadMul<V> mul(MUL ? J.q : 3u, J.mul_r2);
+    const LoadMul<M> mul(MUL ? J.q : 3u, J.mul_r2);
     constexpr int UN = kUnroll * 4 / (int)(sizeof(Sin) * (MUL ? 2 : 1));
     for (uint32_t bb = 0; bb < total; bb += UN * sweep) {
         Sin v[UN][PER];
@@ -884,20 +941,20 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
     if (LR) {  // shape-specialised launches are full tiles with vector loads
         // (the fused product's first pass exists only for uint64 data)
         if constexpr (std::is_same<Sin, uint64_t>::value) {
-            if (flags & JF_MUL_IN) load_loop<4, false, LR, LC, Sin, V, true>(J, poly, col0, pre);
-            else load_loop<4, false, LR, LC, Sin, V>(J, poly, col0, pre);
+            if (flags & JF_MUL_IN) load_loop<4, false, LR, LC, Sin, M, true>(J, poly, col0, pre);
+            else load_loop<4, false, LR, LC, Sin, M>(J, poly, col0, pre);
         } else {
-            load_loop<4, false, LR, LC, Sin, V>(J, poly, col0, pre);
+            load_loop<4, false, LR, LC, Sin, M>(J, poly, col0, pre);
         }
     } else if (flags & JF_MUL_IN) {  // pointwise product fused into the load (predicated: rare path)
-        if (flags & JF_VEC4) load_loop<4, true, LR, LC, Sin, V, true>(J, poly, col0, pre);
-        else load_loop<1, true, LR, LC, Sin, V, true>(J, poly, col0, pre);
+        if (flags & JF_VEC4) load_loop<4, true, LR, LC, Sin, M, true>(J, poly, col0, pre);
+        else load_loop<1, true, LR, LC, Sin, M, true>(J, poly, col0, pre);
     } else if (flags & JF_VEC4) {
-        if (pred_l) load_loop<4, true, LR, LC, Sin, V>(J, poly, col0, pre);
-        else load_loop<4, false, LR, LC, Sin, V>(J, poly, col0, pre);
+        if (pred_l) load_loop<4, true, LR, LC, Sin, M>(J, poly, col0, pre);
+        else load_loop<4, false, LR, LC, Sin, M>(J, poly, col0, pre);
     } else {
-        if (pred_l) load_loop<1, true, LR, LC, Sin, V>(J, poly, col0, pre);
-        else load_loop<1, false, LR, LC, Sin, V>(J, poly, col0, pre);
+        if (pred_l) load_loop<1, true, LR, LC, Sin, M>(J, poly, col0, pre);
+        else load_loop<1, false, LR, LC, Sin, M>(J, poly, col0, pre);
     }
     lds_barrier();
     ct_and_store<M, Sout, MAXM, P4, LR, LC, MF>(J, t, poly, col0, full, mat);

@@ -90,6 +90,7 @@ NTT_PASS_INSTANCES_MF(extern, uint32_t, uint64_t)
 NTT_PASS_INSTANCES_MF(extern, uint64_t, uint32_t)
 NTT_PASS_INSTANCES_MF(extern, uint64_t, uint64_t)
 NTT_PASS_INSTANCES(extern, Mod64, uint64_t, uint64_t)
+NTT_PASS_INSTANCES(extern, ModGold, uint64_t, uint64_t)
 NTT_SHAPES(NTT_SHAPE_INSTANCE_EXTERN)
 NTT_SHAPES_U64(NTT_SHAPE_U64_INSTANCE_EXTERN)
 

@@ -53,6 +53,13 @@ __global__ void pointwise_mul62_kernel(uint64_t* r, const uint64_t* a, const uin
     for (; i < n; i += stride) r[i] = mont64(mont64(a[i], b[i], q, qneg_inv), r2, q, qneg_inv);
 }
 
+// Goldilocks pointwise product r = a*b mod q (q = 2^64 - 2^32 + 1).
+__global__ void pointwise_mulgold_kernel(uint64_t* r, const uint64_t* a, const uint64_t* b, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) r[i] = LoadMul<ModGold>(0, 0)(a[i], b[i]);
+}
+
 __global__ void bitrev_kernel(uint64_t* d, uint64_t n, uint32_t log_n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -129,7 +136,11 @@ __global__ void powers_kernel(uint64_t* out, uint64_t n, const TwP<typename M::V
 static const uint32_t kLdsMax = kPassLdsMax;
 
 // ------------------------------------------------------------------ plans
-enum class Arith { M32, M64 };
+// M32: q < 2^31; M64: q < 2^62 (Shoup); GOLD: q = 2^64 - 2^32 + 1.
+enum class Arith { M32, M64, GOLD };
+static Arith arith_of(uint64_t q) {
+    return q < (1ull << 31) ? Arith::M32 : q == kGoldilocks ? Arith::GOLD : Arith::M64;
+}
 enum class Store { U32, U64 };
 
 static uint32_t vbytes(Arith a) { return a == Arith::M32 ? 4u : 8u; }
@@ -238,7 +249,8 @@ struct Plan {
 // Validates (q, n) the way the transforms need: prime q, 2^log_n | q-1
 // (2^(log_n+1) for the negacyclic root).
 static ntt_status check_modulus(uint64_t q, uint32_t log_n, bool nega) {
-    if (q < 3 || q >= (1ull << 62)) return fail(NTT_ERR_UNSUPPORTED, "modulus must satisfy 3 <= q < 2^62");
+    if (q < 3 || (q >= (1ull << 62) && q != kGoldilocks))
+        return fail(NTT_ERR_UNSUPPORTED, "modulus must satisfy 3 <= q < 2^62 or be 2^64 - 2^32 + 1");
     if (!is_prime(q)) return fail(NTT_ERR_NO_ROOT, "modulus is not prime");
     const uint32_t need = log_n + (nega ? 1u : 0u);
     if (need >= 63 || ((q - 1) & ((1ull << need) - 1)) != 0)
@@ -454,8 +466,10 @@ static bool mfma_ok(uint32_t log_r, Arith ar) { return ar == Arith::M32 && log_r
 //                ends with psi_{2 M_1}^(-(L_{P-1}/R_0) k - (c >> log R_0)).
 // With P = 1 the inverse scales by n^-1 (cyclic) or psi^-k n^-1 (negacyclic).
 // Checked against ntt_host in emulation of these exact index formulas.
+// root: a primitive n-th root of unity to use instead of the generator rule's
+// (cyclic transforms only; 0 = the generator rule).
 static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Arith arith, const Split& sp,
-                             std::unique_ptr<Plan>& out) {
+                             std::unique_ptr<Plan>& out, uint64_t root = 0) {
     ntt_status st = check_modulus(q, log_n, nega);
     if (st != NTT_OK) return st;
     if (arith == Arith::M32 && q >= (1ull << 31)) return fail(NTT_ERR_UNSUPPORTED, "32-bit arithmetic needs q < 2^31");
@@ -472,8 +486,10 @@ static ntt_status build_plan(uint64_t q, uint32_t log_n, bool nega, bool inv, Ar
     P->split = sp;
     P->npass = (int)sp.npass;
     const uint64_t N = 1ull << log_n;
+    if (root && (nega || root >= q || powmod(root, N, q) != 1 || powmod(root, N / 2, q) != q - 1))
+        return fail(NTT_ERR_INVALID_ARG, "root is not a primitive n-th root of unity (or the transform is negacyclic)");
     const uint64_t g = generator(q);
-    const uint64_t omega = powmod(g, (q - 1) / N, q);
+    const uint64_t omega = root ? root : powmod(g, (q - 1) / N, q);
     const uint64_t psi = nega ? powmod(g, (q - 1) / (2 * N), q) : 1;
     const uint64_t w = inv ? invmod(omega, q) : omega;
     const uint64_t p = inv ? invmod(psi, q) : psi;
@@ -765,7 +781,8 @@ static ntt_status launch_generic(Arith ar, Store sin, Store sout, const PassJob*
                 : launch_pass_kernel<Mod32, uint64_t, uint64_t, P4, 0, 0, MF>(djobs, njobs, inl, nblocks, lds, stream);
     } else {
         if (MF) return fail(NTT_ERR_DEVICE, "internal: matrix-core rounds need 32-bit arithmetic");
-        e = launch_pass_kernel<Mod64, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
+        e = ar == Arith::GOLD ? launch_pass_kernel<ModGold, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream)
+                              : launch_pass_kernel<Mod64, uint64_t, uint64_t, P4>(djobs, njobs, inl, nblocks, lds, stream);
     }
     HIP_TRY(e);
     return NTT_OK;
@@ -1042,6 +1059,9 @@ static ntt_status init_config(ntt_c_config* C, ConfigImpl* I) {
         if (I->arith == Arith::M32)
             hipLaunchKernelGGL(powers_kernel<Mod32>, dim3(blocks), dim3(256), 0, 0, dst, n,
                                (const TwP<uint32_t>*)((char*)raw + lo), (const TwP<uint32_t>*)((char*)raw + hi), bits, q);
+        else if (I->arith == Arith::GOLD)
+            hipLaunchKernelGGL(powers_kernel<ModGold>, dim3(blocks), dim3(256), 0, 0, dst, n,
+                               (const TwP<uint64_t>*)((char*)raw + lo), (const TwP<uint64_t>*)((char*)raw + hi), bits, q);
         else
             hipLaunchKernelGGL(powers_kernel<Mod64>, dim3(blocks), dim3(256), 0, 0, dst, n,
                                (const TwP<uint64_t>*)((char*)raw + lo), (const TwP<uint64_t>*)((char*)raw + hi), bits, q);
@@ -1067,7 +1087,7 @@ extern "C" ntt_status ntt_c_init(uint64_t n, uint64_t q, ntt_c_config** out) {
     C->mu = (uint64_t)(((u128)1 << 64) / q);
     ntt_c_create_params(q, n, &C->params);
     ConfigImpl* I = new ConfigImpl();
-    I->arith = q < (1ull << 31) ? Arith::M32 : Arith::M64;
+    I->arith = arith_of(q);
     C->impl = I;
     st = init_config(C, I);
     if (st == NTT_OK) {
@@ -1210,6 +1230,8 @@ static ntt_status pointwise(uint64_t* r, const uint64_t* a, const uint64_t* b, u
     if (q < (1ull << 31)) {
         hipLaunchKernelGGL(pointwise_mul_kernel<uint64_t>, dim3(blocks), dim3(256), 0, stream, r, a, b, n, q,
                            (uint64_t)(((u128)1 << 64) / q));
+    } else if (q == kGoldilocks) {
+        hipLaunchKernelGGL(pointwise_mulgold_kernel, dim3(blocks), dim3(256), 0, stream, r, a, b, n);
     } else {
         uint64_t inv = 1;  // q^-1 mod 2^64 by Newton iteration
         for (int i = 0; i < 6; i++) inv *= 2 - q * inv;
@@ -1350,7 +1372,7 @@ extern "C" ntt_status ntt_c_host(uint64_t* data, uint64_t n, const ntt_c_modular
             uint64_t w = 1;
             for (uint64_t p = 0; p < half; p++) {
                 const uint64_t u = data[g + p], v = mulmod(data[g + p + half], w, q);
-                data[g + p] = (u + v) % q;
+                data[g + p] = (uint64_t)(((u128)u + v) % q);  // exact for q >= 2^63 too
                 data[g + p + half] = u >= v ? u - v : u + q - v;
                 w = mulmod(w, wb, q);
             }
@@ -1373,7 +1395,7 @@ struct ntt_c_context {
 
 static Arith arith_for(uint64_t q, uint32_t flags) {
     (void)flags;
-    return q < (1ull << 31) ? Arith::M32 : Arith::M64;
+    return arith_of(q);
 }
 
 // Plan for a descriptor; `elems_in_flight` (elements transformed by the same

@@ -112,6 +112,10 @@ def _sigs(L):
         "ntt_c_arb_inverse": [vp, vp, u32, u32, P, u32, P, u32, vp],
         "ntt_c_arb_pointwise_mul": [vp, vp, u32, P, u32, u32, vp],
         "ntt_c_arb_scale": [vp, u32, P, P, u32, u32, vp],
+        "ntt_c_arb_plan_create": [ctypes.POINTER(vp), P, u32, u32, u32, P],
+        "ntt_c_arb_plan_forward": [vp, vp, vp],
+        "ntt_c_arb_plan_inverse": [vp, vp, vp],
+        "ntt_c_arb_plan_destroy": [vp],
     }.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -200,6 +204,32 @@ class ArbNtt:
             raise ValueError("built without psi")
         self.inverse(buf, stream)
         pointwise_mul(self.f, buf, self.d_psi_inv, self.n, stream)
+
+
+class ArbPlan:
+    """The transforms of an ArbNtt on the four-step pass kernels
+    (ntt_c_arb_plan_*): moduli that fit 64-bit arithmetic (K <= 2, or K = 3
+    below 2^62 or Goldilocks).  Same data and results as ArbNtt.forward /
+    inverse.  Raises NttError (status NTT_ERR_UNSUPPORTED) for other moduli."""
+
+    def __init__(self, ntt):
+        self.ntt = ntt
+        f = ntt.f
+        root = f.limbs(ntt.omega * f.R)
+        h = ctypes.c_void_p()
+        _check(_n.lib().ntt_c_arb_plan_create(ctypes.byref(h), _u32p(f.p_limbs), f.np, f.K, ntt.log_n, _u32p(root)))
+        self.h = h.value
+
+    def forward(self, buf, stream=None):
+        _check(_n.lib().ntt_c_arb_plan_forward(self.h, buf.ptr, _n._sp(stream)))
+
+    def inverse(self, buf, stream=None):
+        _check(_n.lib().ntt_c_arb_plan_inverse(self.h, buf.ptr, _n._sp(stream)))
+
+    def destroy(self):
+        if self.h:
+            _check(_n.lib().ntt_c_arb_plan_destroy(self.h))
+            self.h = None
 
 
 def pointwise_mul(field, a, b, n, stream=None):

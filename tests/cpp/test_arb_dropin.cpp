@@ -71,6 +71,31 @@ static void roundtrip(int log_n) {
     teardown_ntt<K>(B);
 }
 
+// run_forward / run_inverse (the four-step pass kernels when setup_ntt made a
+// plan: K <= 3 here) against the limb kernels on the same data.
+template <int K>
+static void plan_vs_limb_kernels(int log_n, bool expect_plan) {
+    NTTBuffers<K> B = setup_ntt<K>(log_n, 4242);
+    EXPECT((B.plan != nullptr) == expect_plan);
+    const size_t bytes = (size_t)B.n * sizeof(DScalar<K>);
+    DScalar<K>* d2 = nullptr;
+    hip_check_(hipMalloc((void**)&d2, bytes), "hipMalloc");
+    hip_check_(hipMemcpy(d2, B.d_data, bytes, hipMemcpyDeviceToDevice), "d2d");
+    std::vector<DScalar<K>> x(B.n), a(B.n), b(B.n);
+    hip_check_(hipMemcpy(x.data(), B.d_data, bytes, hipMemcpyDeviceToHost), "d2h");
+    run_forward<K>(B, B.d_data);
+    run_forward_ct<K>(d2, B.d_tw, (uint32_t)B.n, log_n, B.prime_d, B.np);
+    hip_check_(hipMemcpy(a.data(), B.d_data, bytes, hipMemcpyDeviceToHost), "d2h");
+    hip_check_(hipMemcpy(b.data(), d2, bytes, hipMemcpyDeviceToHost), "d2h");
+    EXPECT(std::memcmp(a.data(), b.data(), bytes) == 0);
+    run_inverse<K>(B, B.d_data);
+    hip_check_(hipMemcpy(a.data(), B.d_data, bytes, hipMemcpyDeviceToHost), "d2h");
+    EXPECT(std::memcmp(a.data(), x.data(), bytes) == 0);
+    std::printf("run_forward/run_inverse K=%d log_n=%d (%s) done\n", K, log_n, B.plan ? "plan" : "limb kernels");
+    hip_check_(hipFree(d2), "hipFree");
+    teardown_ntt<K>(B);
+}
+
 template <int K>
 static void negacyclic_product(int log_n) {
     NTTBuffers<K> B = setup_ntt<K>(log_n, 777);
@@ -118,6 +143,10 @@ int main(int argc, char** argv) {
     roundtrip<3>(18);
     roundtrip<11>(12);
     roundtrip<2>(14);
+    plan_vs_limb_kernels<2>(20, true);
+    plan_vs_limb_kernels<3>(20, true);
+    plan_vs_limb_kernels<3>(5, true);
+    plan_vs_limb_kernels<11>(10, false);
     negacyclic_product<3>(6);
     negacyclic_product<11>(5);
     std::printf(g_fail ? "FAILED\n" : "ALL PASSED\n");
