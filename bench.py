@@ -556,6 +556,7 @@ def tune_all(ctx, args):
     rec("serve_bb", BB, list(range(12, 23)), 1 << 24)
     rec("serve_62", W.p62_primes()[0], list(range(12, 23)), 1 << 24)
     rec("single_62", W.p62_primes()[0], list(range(12, 23)), 1)
+    rec("single_gold", 0xFFFFFFFF00000001, list(range(12, 23)), 1)  # Goldilocks (and K = 3 limb plans)
     return out
 
 

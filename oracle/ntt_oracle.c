@@ -98,7 +98,9 @@ uint64_t orc_psi(uint64_t q, uint64_t n) {
 static int ilog2_u64(uint64_t n) { int l = 0; while (((uint64_t)1 << l) < n) l++; return l; }
 
 /* ntt_host, src/ntt_basic.cu:412-463: in-place bit reversal, radix-2 DIT with
- * per-stage w_base = omega^(n/2^(s+1)), canonical add/sub, n^-1 scale on inverse. */
+ * per-stage w_base = omega^(n/2^(s+1)), canonical add/sub, n^-1 scale on inverse.
+ * The sum u + v is formed in 128 bits: the same result for q < 2^63, and exact
+ * for the Goldilocks prime, where the reference's uint64 sum would wrap. */
 void orc_ntt_host(uint64_t* data, uint64_t n, const orc_params* prm, int inverse) {
     int log_n = ilog2_u64(n);
     uint64_t q = prm->q;
@@ -117,8 +119,8 @@ void orc_ntt_host(uint64_t* data, uint64_t n, const orc_params* prm, int inverse
                 uint64_t i = group * full + pos, j = i + half;
                 uint64_t u = data[i];
                 uint64_t v = (uint64_t)((u128)data[j] * w % q);
-                data[i] = (u + v) % q;
-                data[j] = (u >= v) ? (u - v) : (u + q - v);
+                data[i] = (uint64_t)(((u128)u + v) % q);
+                data[j] = (u >= v) ? (u - v) : (u + q - v);  /* < q: exact modulo 2^64 */
                 w = (uint64_t)((u128)w * w_base % q);
             }
         }
@@ -199,8 +201,8 @@ void orc_ntt_pow_twiddle(uint64_t* a, uint64_t n, uint64_t q, uint64_t omega) {
             for (uint64_t k = 0; k < half; k++) {
                 uint64_t w = orc_mod_pow(omega, k * (n / len), q);
                 uint64_t u = a[s + k], v = (uint64_t)((u128)a[s + k + half] * w % q);
-                a[s + k] = (u + v) % q;
-                a[s + k + half] = (u + q - v) % q;
+                a[s + k] = (uint64_t)(((u128)u + v) % q);
+                a[s + k + half] = (uint64_t)(((u128)u + q - v) % q);
             }
     }
 }

@@ -3,8 +3,8 @@
 bench.py and every caller that loads the table run these exact plans, so each
 entry is run here with its own radix split, tile and matrix-core mask, in all
 four directions (cyclic / negacyclic, forward / inverse), on 32-bit and
-uint64 storage (62-bit entries: uint64 only), bit-exact against the oracle up
-to 2^22.  At 2^23 and 2^24 the output is checked at sampled coefficients
+uint64 storage (62-bit and Goldilocks entries: uint64 only), bit-exact
+against the oracle up to 2^22.  At 2^23 and 2^24 the output is checked at sampled coefficients
 against the defining sums and round-tripped through the same plan's inverse
 (the pattern of the reference's tests/test_ntt.cu:288-301: round trip plus
 GPU against host at every size).
@@ -27,6 +27,8 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 TABLE = os.path.join(REPO, "ntt-tensorcore_amd", "tuned_mi355x.txt")
 BB = 2013265921
 P62 = p62_primes()[0]
+GOLD = 0xFFFFFFFF00000001
+Q_OF = {0: BB, 1: P62, 2: GOLD}  # arithmetic kind of an entry -> the prime its cases use
 EXACT_MAX = 22  # oracle up to 2^22 (about a second per transform)
 KINDS = {"cyc": (False, False), "icyc": (False, True), "nega": (True, False), "inega": (True, True)}
 
@@ -49,7 +51,7 @@ CASES = [(e, kind, u64) for e in ENTRIES for kind in KINDS for u64 in ((False, T
 
 def _id(c):
     (ar, log_n, le, split, log_e, mask), kind, u64 = c
-    return f"{'m32' if ar == 0 else 'm64'}-2^{log_n}@2^{le}-{'x'.join(map(str, split))}-e{log_e}-m{mask}-{kind}-" \
+    return f"{('m32', 'm64', 'gold')[ar]}-2^{log_n}@2^{le}-{'x'.join(map(str, split))}-e{log_e}-m{mask}-{kind}-" \
            f"{'u64' if u64 else 'u32'}"
 
 
@@ -79,13 +81,13 @@ def test_table_is_the_shipped_one():
     for ar, log_n, le, split, log_e, mask in ENTRIES:
         assert sum(split) == log_n and 1 <= len(split) <= 4
         assert 8 <= log_e <= 14 and mask < (1 << len(split))
-        assert ar in (0, 1) and (ar == 0 or mask == 0)
+        assert ar in (0, 1, 2) and (ar == 0 or mask == 0)
 
 
 @pytest.mark.parametrize("case", CASES, ids=[_id(c) for c in CASES])
 def test_shipped_plan(nt, orc, ctx, case):
     (ar, log_n, le, split, log_e, mask), kind, u64 = case
-    q = BB if ar == 0 else P62
+    q = Q_OF[ar]
     n = 1 << log_n
     nega, inv = KINDS[kind]
     x, expect = _inputs(orc, q, log_n, kind)
@@ -117,14 +119,14 @@ def shipped_table(nt):
 
 
 SINGLE = sorted({e[1] for e in ENTRIES if e[0] == 0 and e[1] == e[2]})
+SINGLE_ALL = [(BB, ln) for ln in SINGLE] + [(GOLD, e[1]) for e in ENTRIES if e[0] == 2 and e[1] == e[2]]
 
 
-@pytest.mark.parametrize("log_n", SINGLE)
-def test_reference_abi_with_shipped_table(nt, orc, shipped_table, log_n):
+@pytest.mark.parametrize("q,log_n", SINGLE_ALL)
+def test_reference_abi_with_shipped_table(nt, orc, shipped_table, q, log_n):
     """ntt_c_forward/inverse and the negacyclic pair on the reference's uint64_t
     data, with the plan each n gets from the shipped table."""
     n = 1 << log_n
-    q = BB
     x = orc.random_poly(n, q, 77 + log_n)
     cfg = nt.Config(n, q)
     buf = nt.DeviceBuffer.from_array(x)

@@ -12,6 +12,7 @@ import pytest
 BB = 2013265921  # BabyBear 15*2^27+1
 P30 = 1056440321  # largest 30-bit prime = 1 mod 2^19
 P62 = 4611686018326724609  # largest 62-bit prime = 1 mod 2^23
+GOLD = 0xFFFFFFFF00000001  # Goldilocks 2^64 - 2^32 + 1 (sums above 2^64)
 
 
 def py_dft(x, q, w):
@@ -40,13 +41,23 @@ def test_babybear_field_constants(orc):
     assert orc.psi(BB, 1 << 24) == 1149491290
 
 
-@pytest.mark.parametrize("q", [BB, P30, P62])
+@pytest.mark.parametrize("q", [BB, P30, P62, GOLD])
 @pytest.mark.parametrize("log_n", [1, 2, 5, 8])
 def test_root_orders(orc, q, log_n):
     n = 1 << log_n
     w, s = orc.omega(q, n), orc.psi(q, n)
     assert pow(w, n, q) == 1 and pow(w, n // 2, q) == q - 1
     assert pow(s, 2, q) == w and pow(s, n, q) == q - 1
+
+
+def test_goldilocks_generator_and_large_values(orc):
+    """The g-rule gives 7 for Goldilocks (q - 1 = 2^32 * 3 * 5 * 17 * 257 * 65537);
+    inputs near q make every butterfly sum exceed 2^64."""
+    assert orc.generator(GOLD) == 7
+    n = 64
+    x = np.array([GOLD - 1 - (j % 3) for j in range(n)], dtype=np.uint64)
+    w = orc.omega(GOLD, n)
+    assert list(map(int, orc.ntt(x, GOLD))) == py_dft(x, GOLD, w)
 
 
 def test_params_match_create_params(orc):
@@ -63,7 +74,7 @@ def test_params_match_create_params(orc):
     assert orc.lib().orc_mu(BB) == (1 << 64) // BB
 
 
-@pytest.mark.parametrize("q", [BB, P30, P62])
+@pytest.mark.parametrize("q", [BB, P30, P62, GOLD])
 @pytest.mark.parametrize("log_n", [1, 3, 6, 8])
 def test_ntt_host_equals_textbook_dft(orc, q, log_n):
     n = 1 << log_n
@@ -121,7 +132,7 @@ def test_kat_impulse_constant_and_pure_tone(orc, q):
     assert int(X[m]) == n % q and int(np.count_nonzero(X)) == 1
 
 
-@pytest.mark.parametrize("q", [BB, P30, P62])
+@pytest.mark.parametrize("q", [BB, P30, P62, GOLD])
 @pytest.mark.parametrize("log_n", [1, 4, 7])
 def test_negacyclic_definition(orc, q, log_n):
     n = 1 << log_n
@@ -131,7 +142,7 @@ def test_negacyclic_definition(orc, q, log_n):
     assert np.array_equal(orc.ntt_negacyclic(orc.ntt_negacyclic(x, q), q, inverse=True), x)
 
 
-@pytest.mark.parametrize("q", [BB, P62])
+@pytest.mark.parametrize("q", [BB, P62, GOLD])
 @pytest.mark.parametrize("n", [8, 64, 256])
 def test_negacyclic_product_equals_schoolbook(orc, q, n):
     """INTT_neg(NTT_neg(a) * NTT_neg(b)) == a*b mod (x^n+1) (tests/test_ntt.cu:186-199)."""
