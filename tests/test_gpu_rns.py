@@ -18,8 +18,10 @@ def _residues(rng, mods, n):
     return np.stack([rng.integers(0, m, size=n, dtype=np.uint64) for m in mods]).astype(U32)
 
 
-@pytest.mark.parametrize("L,K", [(1, 1), (3, 2), (8, 8), (24, 24), (32, 64), (5, 40)])
-@pytest.mark.parametrize("n", [1, 257, 1028, 1 << 16])
+# every (ceil(L/16), ceil(K/16)) instance of the kernel: (1,1) (1,2) (1,3) (1,4) (2,1) (2,2) (2,3) (2,4)
+@pytest.mark.parametrize("L,K", [(1, 1), (3, 2), (8, 8), (4, 20), (5, 40), (16, 64), (20, 10), (24, 24), (30, 40),
+                                 (32, 64)])
+@pytest.mark.parametrize("n", [1, 257, 1028, 4099, 4100, 1 << 16])
 def test_bconv_vs_oracle(nt, orc, L, K, n):
     primes = _primes(orc, L + K)
     q, p = primes[:L], primes[L:]
