@@ -209,21 +209,48 @@ def extra_2_20(ctx, stream):
         bt = nt.Batch(ctx, descs)
         ms = timed(lambda: bt.launch(stream), 50, 5, stream)
         t = ms / 50 / 1e3
-        # one more launch on fresh inputs, checked against the oracle
         xs = [W.random_poly(n, BB, 100 + b) for b in range(batch)]
-        for b, x in enumerate(xs):
-            buf.upload(x.astype(np.uint32), b * n * 4)
-        bt.launch(stream)
-        stream.sync()
         expect, _ = _oracle().transform_batch(xs, [BB] * batch, [True] * batch, [False] * batch)
-        ok = all(np.array_equal(buf.download(np.uint32, n, b * n * 4).astype(np.uint64), e)
-                 for b, e in enumerate(expect))
-        _require(ok, f"negacyclic_2_20 batch {batch}: output differs from the oracle")
+
+        def check(form):
+            # one more launch on fresh inputs, checked against the oracle
+            for b, x in enumerate(xs):
+                buf.upload(x.astype(np.uint32), b * n * 4)
+            bt.launch(stream)
+            bt.check(stream)
+            ok = all(np.array_equal(buf.download(np.uint32, n, b * n * 4).astype(np.uint64), e)
+                     for b, e in enumerate(expect))
+            _require(ok, f"negacyclic_2_20 batch {batch} ({form}): output differs from the oracle")
+
+        check("per-phase launches")
         out[f"batch{batch}"] = {"Gbfly_s": batch * W.butterflies(20) / t / 1e9,
                                 "GB_s": batch * 2 * n * 4 / t / 1e9, "us": t * 1e6, "checked": "oracle"}
+        if batch > 1:
+            out[f"batch{batch}"]["single_launch_queue_us"] = _queue_us(bt, stream)
+            check("single work-queue launch")
         bt.destroy()
         buf.free()
     return out
+
+
+def _queue_us(bt, stream):
+    """The batch as one persistent work-queue launch (graph replay), timed,
+    then checked against the oracle by the caller's next check; the batch is
+    left in its default form (one launch per phase group)."""
+    bt.set_queue(True)
+    ms = timed(lambda: bt.launch_graph(stream), 20, 3, stream) / 20
+    bt.check(stream)
+    return ms * 1e3
+
+
+def _queue_checked(bt, buf, offs, jobs, stream, name):
+    """Times the single-launch form and checks its output against the oracle."""
+    us = _queue_us(bt, stream)
+    _, ok = _check_batch(buf, offs, jobs, lambda: bt.launch_graph(stream), stream)
+    bt.check(stream)
+    _require(ok, f"{name}: single-launch work-queue output differs from the oracle")
+    bt.set_queue(False)
+    return us
 
 
 def _pack(jobs, dtype):
@@ -336,11 +363,12 @@ def extra_rns(ctx, stream):
     t = ms / 50 / 1e3
     cpu, ok = _check_batch(buf, offs, jobs, lambda: bt.launch_graph(stream), stream)
     _require(ok, "rns_24x2_16: output differs from the oracle")
+    queue_us = _queue_checked(bt, buf, offs, jobs, stream, "rns_24x2_16")
     bt.destroy()
     buf.free()
     n = 24 << 16
     bfly = 24 * W.butterflies(16)
-    return {"limbs": 24, "log_n": 16, "us": t * 1e6, "Gbfly_s": bfly / t / 1e9,
+    return {"limbs": 24, "log_n": 16, "us": t * 1e6, "single_launch_queue_us": queue_us, "Gbfly_s": bfly / t / 1e9,
             "GB_s": 2 * n * 4 / t / 1e9, "frac": 2 * n * 4 / t / 1e9 / HBM_PEAK_GBS, "checked": "oracle",
             "cpu_baseline": _cpu_pool(bfly, cpu, "24 limbs x 2^16 negacyclic forward")}
 
@@ -404,9 +432,11 @@ def extra_uneven(ctx, stream):
     _require(ok, "uneven_512: per-polynomial output differs from the oracle")
     _, ok = _check_batch(buf, offs, jobs, per_poly_py, stream)
     _require(ok, "uneven_512: per-polynomial (Python loop) output differs from the oracle")
+    queue_us = _queue_checked(bt, buf, offs, jobs, stream, "uneven_512")
     bt.destroy()
     buf.free()
-    return {"polys": len(jobs), "batched_us": ms_b * 1e3, "per_poly_us": ms_p * 1e3,
+    return {"polys": len(jobs), "batched_us": ms_b * 1e3, "single_launch_queue_us": queue_us,
+            "per_poly_us": ms_p * 1e3,
             "speedup": ms_p / ms_b, "per_poly_python_loop_us": ms_py * 1e3,
             "speedup_vs_python_loop": ms_py / ms_b, "batched_Gbfly_s": bfly / (ms_b / 1e3) / 1e9,
             "batched_GB_s": byt / (ms_b / 1e3) / 1e9, "launches": info["launches"],

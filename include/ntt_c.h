@@ -222,14 +222,37 @@ ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc* desc, uin
  * workspace.  The descriptors' data pointers are bound at prepare time. */
 ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count, ntt_c_batch** out);
 /* Launch a prepared batch: one kernel launch per (pass, arithmetic, storage
- * pair, round kind, post-multiply kind) group of its jobs, so the launch
- * count depends on the kinds in the mix, not on the number of polynomials
- * (ntt_c_batch_info reports it). */
+ * pair, round kind) group of its jobs (two launches for a 32-bit mix of one-
+ * and two-pass plans), or, after ntt_c_batch_set_queue(batch, 1, 0), the
+ * whole batch as ONE persistent launch.  Either way the launch count depends
+ * on the kinds in the mix, not on the number of polynomials
+ * (ntt_c_batch_info reports it).  Replaces the reference's per-polynomial
+ * loop (src/ntt_basic.cu:337-389). */
 ntt_status ntt_c_batch_launch(ntt_c_batch* batch, void* stream);
-/* Capture the batch's launches into a hipGraph once; later calls replay it. */
+/* Capture the batch's launches into a hipGraph once; later calls replay it
+ * (a new capture after ntt_c_batch_set_queue changes the launch form). */
 ntt_status ntt_c_batch_launch_graph(ntt_c_batch* batch, void* stream);
+/* Choose the launch form of a prepared batch (no reference counterpart):
+ * enable = 1, the single persistent work-queue launch: workgroups take tiles
+ * from per-XCD queues by atomic tickets and a pass of a polynomial starts as
+ * soon as that polynomial's previous pass is complete (device counters), each
+ * polynomial kept on one XCD where a probe launch saw all XCDs (else
+ * agent-scope fences hand the data over); 2, the same launch with agent-scope
+ * fences in every case; 0, one launch per phase group (the default: it
+ * measured faster on every batch workload).  The work queue needs every job
+ * on one kernel instantiation (32-bit coefficients and arithmetic, one round
+ * kind): NTT_ERR_UNSUPPORTED otherwise.
+ * spin_limit bounds each wait of a dependent tile (0: the default, about a
+ * second); a wait that gives up is reported by ntt_c_batch_check.  Calls that
+ * overlap on one batch race on its device counters. */
+ntt_status ntt_c_batch_set_queue(ntt_c_batch* batch, int enable, uint32_t spin_limit);
+/* Synchronises `stream`, then returns NTT_ERR_DEVICE (and clears the flag) if a
+ * work-queue wait of an earlier launch of this batch gave up before its inputs
+ * were complete, else NTT_OK. */
+ntt_status ntt_c_batch_check(ntt_c_batch* batch, void* stream);
 ntt_status ntt_c_batch_destroy(ntt_c_batch* batch);
-/* Launch counts and workspace of a prepared batch. */
+/* Launch count, workspace bytes and launched workgroups of a prepared batch
+ * in its current launch form. */
 ntt_status ntt_c_batch_info(const ntt_c_batch* batch, uint32_t* launches, uint64_t* workspace_bytes,
                             uint64_t* workgroups);
 

@@ -256,6 +256,12 @@ struct PassJob {
     // ever multiplies a zero term, so its shift is arbitrary)
     uint8_t sh_sr, sh_sc, sh_or, sh_oc;
     uint8_t sh_ohs, pad_[3];
+    // Work-queue launches (queue_kernel): the tiles of this job start once
+    // dep_tiles tiles of job `dep` (the previous pass of the same polynomial)
+    // have completed; dep_tiles = 0: no dependency.  counted: the job's tiles
+    // count their completions (a later job depends on it, or it has a
+    // dependency itself).
+    uint32_t dep, dep_tiles, counted, pad2_;
 };
 
 // Threads of every pass-kernel workgroup (a compile-time constant so that the
@@ -1000,6 +1006,136 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(M::kMin
     }
     const PassJob J = jobs[find_job(jobs, njobs, tile)];
     run_pass<M, Sin, Sout, MAXM, P4, LR, LC, MF>(J, tile - J.tile_begin);
+}
+
+// ---- a whole multi-pass batch in one persistent launch ----
+//
+// The jobs (one per polynomial and pass) are numbered in one tile space and
+// split into nq queues, queue x holding the tiles [qbeg[x], qbeg[x+1]) in an
+// order where every job comes after the job it depends on (tjob[g]: the job
+// of tile g).  A workgroup takes tiles of its queue by atomic tickets; before
+// a tile of a dependent job its thread 0 waits until the completion counter of
+// the job it depends on shows all of that job's tiles done in this launch.
+// Tickets go out in queue order to running workgroups only, so every wait is
+// for tiles that running workgroups hold and the launch cannot deadlock,
+// whatever the grid size or the dispatch order; each wait is also bounded
+// (spin_limit polls; status bit 0 records one that gave up, whose tile then
+// ran without its inputs complete).
+//
+// Completion counters are 64-bit and never reset: job j's counter has gained
+// tiles(j) in every earlier launch, so a tile of a counted job computes the
+// launch's generation G = counter(j) / tiles(j) (its own tile is not counted
+// yet) and waits for counter(dep) >= (G + 1) * tiles(dep).  Tickets are
+// reset by the last workgroup to leave (a relaxed exit counter; every
+// workgroup's ticket atomics have returned before it counts itself out).
+// Launches of one batch must not overlap: they share the counters.
+//
+// Two hand-over protocols (QueueProto):
+//   QP_AGENT: queue = blockIdx mod nq, and a workgroup whose queue is empty
+//     takes tiles of the next one.  The grid-barrier pattern at agent scope:
+//     the producer's waves finish their stores, the workgroup barrier orders
+//     them before thread 0's agent-scope fence and counter increment; the
+//     consumer's thread 0 sees the count and fences, and the barrier orders
+//     the workgroup's loads after it.  Correct on any dispatch, but each
+//     fence writes back and invalidates the XCD's L2.
+//   QP_XCD: queue = the XCD the workgroup runs on (hardware register
+//     XCC_ID; nq = 8), and a workgroup only takes tiles of its own queue, so
+//     all passes of a polynomial run on one XCD and its data moves through
+//     that XCD's L2, which all its CUs share.  The producer's waves wait until
+//     their stores are acknowledged (s_waitcnt vmcnt(0): the data is in the
+//     L2) before the barrier and the counter increment; the consumer
+//     invalidates its CU's L1 after the wait (buffer_inv sc0), so its loads
+//     read the L2.  The host uses it after a probe launch has seen workgroups
+//     on all 8 XCDs; a queue no workgroup drained is reported (status bit 1).
+//
+// ctl (64-bit words, each on its own 256-byte line so that polling and
+// counting on one line do not serialise): status, exit count, tickets[nq],
+// then the completion counters.
+// tslot: LDS byte offset of two words past every job's tile and tables.
+enum QueueProto : uint32_t { QP_AGENT = 0, QP_XCD = 1 };
+constexpr uint32_t kQLine = 32;  // 64-bit words per 256-byte line
+constexpr uint32_t kQStatus = 0, kQExits = kQLine;
+__host__ __device__ constexpr uint32_t q_ticket(uint32_t x) { return (2u + x) * kQLine; }
+__host__ __device__ constexpr uint32_t q_done(uint32_t j) { return (10u + j) * kQLine; }
+
+__device__ __forceinline__ uint32_t hw_xcc_id() { return __builtin_amdgcn_s_getreg(6164) & 15u; }  // hwreg(HW_REG_XCC_ID, 0, 4)
+
+template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, bool MF>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(M::kMinWaves > 6 ? M::kMinWaves : 6)))
+queue_kernel(const PassJob* __restrict__ jobs, const uint32_t* __restrict__ tjob, const uint32_t* __restrict__ qbeg,
+             uint32_t nq, unsigned long long* ctl, uint32_t tslot, uint32_t spin_limit, uint32_t proto) {
+    const uint32_t tid = threadIdx.x;
+    const bool xcd = proto == QP_XCD;
+    unsigned int* status = reinterpret_cast<unsigned int*>(ctl + kQStatus);
+    // thread 0's queue cursor; QP_XCD never leaves its own queue
+    uint32_t x = (xcd ? hw_xcc_id() : blockIdx.x) % nq, tried = xcd ? nq - 1u : 0u;
+    for (;;) {
+        if (tid == 0) {
+            uint32_t g = 0xFFFFFFFFu, j = 0;
+            while (tried < nq) {
+                const uint32_t k = (uint32_t)__hip_atomic_fetch_add(ctl + q_ticket(x), 1ull, __ATOMIC_RELAXED,
+                                                                    __HIP_MEMORY_SCOPE_AGENT);
+                if (k < qbeg[x + 1] - qbeg[x]) {
+                    g = qbeg[x] + k;
+                    break;
+                }
+                x = x + 1 == nq ? 0 : x + 1;
+                tried++;
+            }
+            if (g != 0xFFFFFFFFu) {
+                j = tjob[g];
+                const PassJob& J = jobs[j];
+                if (J.dep_tiles) {
+                    const unsigned long long gen =
+                        __hip_atomic_load(ctl + q_done(j), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) /
+                        (J.tiles_per_poly * (J.npoly ? J.npoly : 1u));
+                    const unsigned long long need = (gen + 1) * J.dep_tiles;
+                    const unsigned long long* c = ctl + q_done(J.dep);
+                    uint32_t polls = 0;
+                    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                        __builtin_amdgcn_s_sleep(4);
+                        if (++polls >= spin_limit) {
+                            __hip_atomic_fetch_or(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                    }
+                    if (xcd) __asm__ volatile("buffer_inv sc0" ::: "memory");
+                    else __threadfence();
+                }
+            }
+            lds_st<uint32_t>(tslot, g);
+            lds_st<uint32_t>(tslot + 4u, j);
+        }
+        __syncthreads();
+        const uint32_t g = lds_ld<uint32_t>(tslot), j = lds_ld<uint32_t>(tslot + 4u);
+        if (g == 0xFFFFFFFFu) break;
+        const PassJob J = jobs[j];
+        run_pass<M, Sin, Sout, MAXM, P4, 0, 0, MF>(J, g - J.tile_begin);
+        if (J.counted) {
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are acknowledged
+            __syncthreads();
+            if (tid == 0) {
+                if (!xcd) __threadfence();
+                __hip_atomic_fetch_add(ctl + q_done(j), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else {
+            __syncthreads();  // the tile's LDS is free again
+        }
+    }
+    if (tid == 0) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // this wave's counter increments have landed
+        const unsigned long long e =
+            __hip_atomic_fetch_add(ctl + kQExits, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e + 1 == gridDim.x) {  // every other workgroup has taken its last ticket
+            for (uint32_t q = 0; q < nq; q++) {
+                const unsigned long long t =
+                    __hip_atomic_load(ctl + q_ticket(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t < qbeg[q + 1] - qbeg[q]) __hip_atomic_fetch_or(status, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(ctl + q_ticket(q), 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __hip_atomic_store(ctl + kQExits, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 // ---- small elementwise kernels ----

@@ -34,6 +34,44 @@ hipError_t launch_pass_kernel(const PassJob* djobs, uint32_t njobs, const PassJo
     return hipGetLastError();
 }
 
+// Launches queue_kernel<M, Sin, Sout, 4, kPassThreads, P4, MF> (one launch
+// for every pass of a batch; ntt_kernels.hpp) on nblocks workgroups.
+template <class M, class Sin, class Sout, bool P4, bool MF>
+hipError_t launch_queue_kernel(const PassJob* djobs, const uint32_t* tjob, const uint32_t* qbeg, uint32_t nq,
+                               unsigned long long* ctl, uint32_t tslot, uint32_t spin_limit, uint32_t proto,
+                               uint32_t nblocks, uint32_t lds, hipStream_t stream) {
+    auto fn = queue_kernel<M, Sin, Sout, 4, kPassThreads, P4, MF>;
+    static std::once_flag once;
+    static hipError_t ready = hipSuccess;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPassLdsMax);
+        hipFuncAttributes at{};
+        ready = hipFuncGetAttributes(&at, (const void*)fn);
+        if (ready == hipSuccess && at.sharedSizeBytes != 0) ready = hipErrorInvalidKernelFile;
+    });
+    if (ready != hipSuccess) return ready;
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(kPassThreads), lds, stream, djobs, tjob, qbeg, nq, ctl, tslot,
+                       spin_limit, proto);
+    return hipGetLastError();
+}
+
+// Workgroups of queue_kernel<...> that fit on one CU with `lds` bytes of LDS.
+template <class M, class Sin, class Sout, bool P4, bool MF>
+int queue_kernel_per_cu(uint32_t lds) {
+    auto fn = queue_kernel<M, Sin, Sout, 4, kPassThreads, P4, MF>;
+    (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPassLdsMax);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kPassThreads, lds) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
+}
+
+#define NTT_QUEUE_INSTANCE_1(EXT, MF)                                                                          \
+    EXT template hipError_t launch_queue_kernel<Mod32, uint32_t, uint32_t, true, MF>(                           \
+        const PassJob*, const uint32_t*, const uint32_t*, uint32_t, unsigned long long*, uint32_t, uint32_t,    \
+        uint32_t, uint32_t, uint32_t, hipStream_t);                                                              \
+    EXT template int queue_kernel_per_cu<Mod32, uint32_t, uint32_t, true, MF>(uint32_t);
+#define NTT_QUEUE_INSTANCES(EXT) NTT_QUEUE_INSTANCE_1(EXT, false) NTT_QUEUE_INSTANCE_1(EXT, true)
+
 #define NTT_PASS_INSTANCE_1(EXT, M, SIN, SOUT, P4, MF)                                                         \
     EXT template hipError_t launch_pass_kernel<M, SIN, SOUT, P4, 0, 0, MF>(const PassJob*, uint32_t,           \
                                                                           const PassJob&, uint32_t, uint32_t,  \
@@ -93,5 +131,6 @@ NTT_PASS_INSTANCES(extern, Mod64, uint64_t, uint64_t)
 NTT_PASS_INSTANCES(extern, ModGold, uint64_t, uint64_t)
 NTT_SHAPES(NTT_SHAPE_INSTANCE_EXTERN)
 NTT_SHAPES_U64(NTT_SHAPE_U64_INSTANCE_EXTERN)
+NTT_QUEUE_INSTANCES(extern)
 
 }  // namespace nttk

@@ -1454,20 +1454,177 @@ struct ntt_c_batch {
         PassJob inl{};
     };
     std::vector<Group> phases[4];  // phase k: pass k of every polynomial
+    // The whole batch as one work-queue launch (queue_kernel), when every job
+    // runs on one kernel instantiation (32-bit arithmetic and storage, one
+    // round kind): device block = jobs, queue bounds, control words.
+    struct Queue {
+        bool ok = false;
+        bool mf = false;
+        void* dev = nullptr;
+        PassJob* djobs = nullptr;
+        uint32_t* tjob = nullptr;  // job of every global tile
+        uint32_t* qbeg = nullptr;
+        unsigned long long* ctl = nullptr;  // status word, completion counters (queue_kernel)
+        uint32_t njobs = 0, nq = 0, nblocks = 0, lds = 0, tslot = 0, tiles = 0;
+        uint32_t spin_limit = 1u << 22;
+        uint32_t proto = QP_AGENT;      // protocol in use
+        uint32_t proto_auto = QP_AGENT; // what the device probe allows
+    } queue;
+    bool use_queue = false;
+    int graph_mode = -1;           // use_queue of the captured graph
     void* work = nullptr;
     uint64_t work_bytes = 0;
     uint64_t workgroups = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    ~ntt_c_batch() {
+    void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_mode = -1;
+    }
+    ~ntt_c_batch() {
+        drop_graph();
         for (auto& ph : phases)
             for (auto& g : ph)
                 if (g.djobs) (void)hipFree(g.djobs);
+        if (queue.dev) (void)hipFree(queue.dev);
         if (work) (void)hipFree(work);
     }
 };
+
+// Work-queue plan of a batch (queue_kernel in ntt_kernels.hpp): polynomials
+// are spread over nq queues by longest-processing-time first (one queue per
+// XCD, so a polynomial's passes share an L2); within a queue the jobs go
+// phase by phase, each job after the one it depends on.  items[i] holds the
+// jobs of polynomial i in pass order.  Returns false (the batch keeps its
+// per-phase launches) when the jobs need more than one kernel instantiation.
+struct QueueItem {
+    PassJob job;
+    Arith ar;
+    Store sin, sout;
+};
+
+// Writes the XCC_ID of every workgroup (the queue protocol's probe).
+__global__ void xcc_probe_kernel(uint32_t* out) {
+    if (threadIdx.x == 0) out[blockIdx.x] = hw_xcc_id();
+}
+
+// Whether the XCD-local queue protocol can run on `dev`: a probe launch of
+// 64 * nq workgroups must see XCC_ID values below nq on every workgroup and
+// every one of the nq XCDs in use.  Cached per device.
+static bool xcd_queues_ok(int dev, uint32_t nq) {
+    static std::mutex mu;
+    static std::map<int, bool> known;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(dev);
+    if (it != known.end()) return it->second;
+    bool ok = false;
+    const uint32_t nb = 64u * nq;
+    uint32_t* d = nullptr;
+    if (nq == 8 && hipMalloc(&d, nb * sizeof(uint32_t)) == hipSuccess) {
+        std::vector<uint32_t> h(nb, 0xFFu);
+        hipLaunchKernelGGL(xcc_probe_kernel, dim3(nb), dim3(64), 0, 0, d);
+        if (hipDeviceSynchronize() == hipSuccess &&
+            hipMemcpy(h.data(), d, nb * sizeof(uint32_t), hipMemcpyDeviceToHost) == hipSuccess) {
+            std::vector<bool> seen(nq, false);
+            ok = true;
+            for (uint32_t b = 0; b < nb; b++) {
+                ok = ok && h[b] < nq;
+                if (h[b] < nq) seen[h[b]] = true;
+            }
+            for (uint32_t x = 0; x < nq; x++) ok = ok && seen[x];
+        }
+        (void)hipFree(d);
+    }
+    (void)hipGetLastError();
+    known[dev] = ok;
+    return ok;
+}
+static ntt_status build_queue(ntt_c_batch::Queue& Q, const std::vector<std::vector<QueueItem>>& polys) {
+    Q.ok = false;
+    if (polys.empty()) return NTT_OK;
+    const QueueItem& f = polys[0][0];
+    for (auto& pv : polys)
+        for (auto& it : pv)
+            if (it.ar != Arith::M32 || it.sin != Store::U32 || it.sout != Store::U32 ||
+                launch_key(it.job) != launch_key(f.job) || it.job.npoly > 1)
+                return NTT_OK;
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t nq = (cus >= 8 && cus % 8 == 0) ? 8u : 1u;
+    // longest processing time first: elements moved over all passes
+    std::vector<size_t> order(polys.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    auto cost = [&](size_t i) {
+        uint64_t c = 0;
+        for (auto& it : polys[i]) c += (uint64_t)job_tiles(it.job) << (it.job.log_r + it.job.log_c);
+        return c;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+    std::vector<uint64_t> load(nq, 0);
+    std::vector<std::vector<size_t>> members(nq);
+    for (size_t i : order) {
+        const size_t x = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        members[x].push_back(i);
+        load[x] += cost(i);
+    }
+    std::vector<PassJob> jobs;
+    std::vector<uint32_t> qbeg(nq + 1, 0);
+    std::vector<int64_t> last(polys.size(), -1);  // index of the poly's previous-pass job
+    uint32_t tiles = 0, lds = 0;
+    for (uint32_t x = 0; x < nq; x++) {
+        qbeg[x] = tiles;
+        for (int k = 0; k < 4; k++)
+            for (size_t i : members[x]) {
+                if ((size_t)k >= polys[i].size()) continue;
+                PassJob J = polys[i][k].job;
+                J.tile_begin = tiles;
+                J.dep = last[i] < 0 ? 0u : (uint32_t)last[i];
+                J.dep_tiles = last[i] < 0 ? 0u : job_tiles(jobs[(size_t)last[i]]);
+                J.counted = last[i] >= 0 ? 1u : 0u;
+                if (last[i] >= 0) jobs[(size_t)last[i]].counted = 1;
+                last[i] = (int64_t)jobs.size();
+                tiles += job_tiles(J);
+                lds = std::max(lds, job_lds(J, Arith::M32));
+                jobs.push_back(J);
+            }
+    }
+    qbeg[nq] = tiles;
+    if (tiles == 0) return NTT_OK;
+    Q.mf = launch_key(f.job) != 0;
+    Q.njobs = (uint32_t)jobs.size();
+    Q.nq = nq;
+    Q.tiles = tiles;
+    Q.tslot = (lds + 15u) & ~15u;  // two words: the tile taken and its job
+    Q.lds = Q.tslot + 16u;
+    if (Q.lds > kPassLdsMax) return NTT_OK;
+    const int per_cu = Q.mf ? queue_kernel_per_cu<Mod32, uint32_t, uint32_t, true, true>(Q.lds)
+                            : queue_kernel_per_cu<Mod32, uint32_t, uint32_t, true, false>(Q.lds);
+    Q.nblocks = std::min<uint32_t>(tiles, (uint32_t)(per_cu * cus));
+    Q.proto_auto = xcd_queues_ok(dev, nq) ? QP_XCD : QP_AGENT;
+    Q.proto = Q.proto_auto;
+    std::vector<uint32_t> tjob(tiles);
+    for (size_t j = 0; j < jobs.size(); j++)
+        for (uint32_t t = 0; t < job_tiles(jobs[j]); t++) tjob[jobs[j].tile_begin + t] = (uint32_t)j;
+    const size_t jb = jobs.size() * sizeof(PassJob), tb = (size_t)tiles * sizeof(uint32_t),
+                 qb = (nq + 1) * sizeof(uint32_t), cb = q_done((uint32_t)jobs.size()) * sizeof(unsigned long long);
+    const size_t toff = (jb + 255) & ~(size_t)255, qoff = (toff + tb + 255) & ~(size_t)255,
+                 coff = (qoff + qb + 255) & ~(size_t)255;
+    if (hipMalloc(&Q.dev, coff + cb) != hipSuccess) return fail(NTT_ERR_OOM, "batch_prepare: queue allocation failed");
+    Q.djobs = (PassJob*)Q.dev;
+    Q.tjob = (uint32_t*)((char*)Q.dev + toff);
+    Q.qbeg = (uint32_t*)((char*)Q.dev + qoff);
+    Q.ctl = (unsigned long long*)((char*)Q.dev + coff);
+    HIP_TRY(hipMemcpy(Q.djobs, jobs.data(), jb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(Q.tjob, tjob.data(), tb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(Q.qbeg, qbeg.data(), qb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(Q.ctl, 0, cb));
+    Q.ok = true;
+    return NTT_OK;
+}
 
 extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count,
                                           ntt_c_batch** out) {
@@ -1476,6 +1633,7 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
     auto B = std::make_unique<ntt_c_batch>();
     struct Item { PassJob job; Arith ar; Store sin, sout; int phase; };
     std::vector<Item> items;
+    std::vector<std::vector<QueueItem>> qpolys(count);
     std::vector<const Plan*> plans(count);
     uint64_t work_bytes = 0, elems = 0;
     for (size_t i = 0; i < count; i++) {
@@ -1506,7 +1664,18 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
         PassJob jobs[4];
         Store io[4][2];
         make_jobs(P, descs[i].data, st, 1, scr, jobs, io);
-        for (int k = 0; k < P.npass; k++) items.push_back({jobs[k], P.arith, io[k][0], io[k][1], k});
+        for (int k = 0; k < P.npass; k++) {
+            items.push_back({jobs[k], P.arith, io[k][0], io[k][1], k});
+            qpolys[i].push_back({jobs[k], P.arith, io[k][0], io[k][1]});
+        }
+    }
+    {
+        // The work queue is built where the mix allows it but not used by
+        // default: the per-phase launches measured faster on every batch
+        // workload (ntt_c_batch_set_queue switches).
+        ntt_status st = build_queue(B->queue, qpolys);
+        if (st != NTT_OK) return st;
+        B->use_queue = false;
     }
     // Group by (phase, arith, storage); within a group order by LDS size so
     // that big tiles start first.
@@ -1553,6 +1722,18 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
 }
 
 static ntt_status batch_enqueue(ntt_c_batch* B, hipStream_t stream) {
+    if (B->use_queue) {
+        const ntt_c_batch::Queue& Q = B->queue;
+        const hipError_t e =
+            Q.mf ? launch_queue_kernel<Mod32, uint32_t, uint32_t, true, true>(Q.djobs, Q.tjob, Q.qbeg, Q.nq, Q.ctl,
+                                                                               Q.tslot, Q.spin_limit, Q.proto,
+                                                                               Q.nblocks, Q.lds, stream)
+                 : launch_queue_kernel<Mod32, uint32_t, uint32_t, true, false>(Q.djobs, Q.tjob, Q.qbeg, Q.nq, Q.ctl,
+                                                                                Q.tslot, Q.spin_limit, Q.proto,
+                                                                                Q.nblocks, Q.lds, stream);
+        HIP_TRY(e);
+        return NTT_OK;
+    }
     for (auto& ph : B->phases)
         for (auto& g : ph) {
             ntt_status st =
@@ -1569,6 +1750,7 @@ extern "C" ntt_status ntt_c_batch_launch(ntt_c_batch* B, void* stream) {
 
 extern "C" ntt_status ntt_c_batch_launch_graph(ntt_c_batch* B, void* stream) {
     if (!B) return fail(NTT_ERR_INVALID_ARG, "batch_launch_graph: null batch");
+    if (B->exec && B->graph_mode != (int)B->use_queue) B->drop_graph();
     if (!B->exec) {
         hipStream_t cap = nullptr;
         HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
@@ -1585,6 +1767,7 @@ extern "C" ntt_status ntt_c_batch_launch_graph(ntt_c_batch* B, void* stream) {
         if (e != hipSuccess) return fail(NTT_ERR_DEVICE, "hipStreamEndCapture failed");
         B->graph = graph;
         HIP_TRY(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
+        B->graph_mode = (int)B->use_queue;
     }
     HIP_TRY(hipGraphLaunch(B->exec, (hipStream_t)stream));
     return NTT_OK;
@@ -1598,15 +1781,44 @@ extern "C" ntt_status ntt_c_batch_destroy(ntt_c_batch* B) {
     return NTT_OK;
 }
 
+extern "C" ntt_status ntt_c_batch_set_queue(ntt_c_batch* B, int enable, uint32_t spin_limit) {
+    if (!B) return fail(NTT_ERR_INVALID_ARG, "batch_set_queue: null batch");
+    if (enable && !B->queue.ok)
+        return fail(NTT_ERR_UNSUPPORTED, "batch_set_queue: this mix needs more than one kernel instantiation");
+    if (enable < 0 || enable > 2) return fail(NTT_ERR_INVALID_ARG, "batch_set_queue: enable must be 0, 1 or 2");
+    B->use_queue = enable != 0;
+    B->queue.spin_limit = spin_limit ? spin_limit : (1u << 22);
+    const uint32_t proto = enable == 2 ? (uint32_t)QP_AGENT : B->queue.proto_auto;
+    if (proto != B->queue.proto) {
+        B->queue.proto = proto;
+        B->drop_graph();
+    }
+    return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_batch_check(ntt_c_batch* B, void* stream) {
+    if (!B) return fail(NTT_ERR_INVALID_ARG, "batch_check: null batch");
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    if (!B->queue.ok) return NTT_OK;
+    uint32_t st = 0;
+    HIP_TRY(hipMemcpy(&st, B->queue.ctl + kQStatus, sizeof(st), hipMemcpyDeviceToHost));
+    if (st) {
+        HIP_TRY(hipMemset(B->queue.ctl + kQStatus, 0, sizeof(st)));
+        return fail(NTT_ERR_DEVICE, (st & 2u) ? "batch: a work queue was left with tiles no workgroup took"
+                                             : "batch: a work-queue wait gave up before its inputs were complete");
+    }
+    return NTT_OK;
+}
+
 extern "C" ntt_status ntt_c_batch_info(const ntt_c_batch* B, uint32_t* launches, uint64_t* ws, uint64_t* wgs) {
     if (!B) return fail(NTT_ERR_INVALID_ARG, "batch_info: null batch");
     if (launches) {
         uint32_t l = 0;
         for (auto& ph : B->phases) l += (uint32_t)ph.size();
-        *launches = l;
+        *launches = B->use_queue ? 1u : l;
     }
     if (ws) *ws = B->work_bytes;
-    if (wgs) *wgs = B->workgroups;
+    if (wgs) *wgs = B->use_queue ? B->queue.nblocks : B->workgroups;
     return NTT_OK;
 }
 

@@ -97,6 +97,8 @@ def lib():
         "ntt_c_batch_prepare": [vp, P(Desc), ctypes.c_size_t, P(vp)],
         "ntt_c_batch_launch": [vp, vp],
         "ntt_c_batch_launch_graph": [vp, vp],
+        "ntt_c_batch_set_queue": [vp, i32, u32],
+        "ntt_c_batch_check": [vp, vp],
         "ntt_c_batch_destroy": [vp],
         "ntt_c_batch_info": [vp, P(u32), P(u64), P(u64)],
         "ntt_c_device_count": [P(i32)],
@@ -475,6 +477,16 @@ class Batch:
 
     def launch_graph(self, stream=None):
         check(lib().ntt_c_batch_launch_graph(self.ptr, _sp(stream)))
+
+    def set_queue(self, enable, spin_limit=0, agent_fences=False):
+        """enable=True: the whole batch in one work-queue launch (the default
+        where the mix allows it), with agent_fences=True forcing the
+        agent-scope hand-over; False: one launch per phase group."""
+        check(lib().ntt_c_batch_set_queue(self.ptr, (2 if agent_fences else 1) if enable else 0, spin_limit))
+
+    def check(self, stream=None):
+        """Synchronises the stream; raises if a work-queue wait gave up."""
+        check(lib().ntt_c_batch_check(self.ptr, _sp(stream)))
 
     def info(self):
         l, w, g = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
