@@ -274,6 +274,14 @@ ntt_status ntt_c_autotune_clear(void);
 /* The split now chosen for 2^log_n with elems_in_flight elements per launch:
  * out = {npass, log_r[0..3], log_e, mfma_mask}. */
 ntt_status ntt_c_split_for(uint64_t q, uint32_t log_n, uint64_t elems_in_flight, uint32_t out[7]);
+/* Process-wide kernel choice (no reference counterpart).  NTT_OPT_PIPELINE:
+ * value 1 runs each pass of a single transform (32-bit storage, matrix-core
+ * rounds, a pipelined tile shape) as workgroups that walk several tiles and
+ * load the next tile while transforming the current one (the default); 0 runs
+ * one tile per workgroup.  Returns the previous value in *old (may be NULL).
+ * Results are identical either way. */
+#define NTT_OPT_PIPELINE 1
+ntt_status ntt_c_set_option(uint32_t option, int64_t value, int64_t* old);
 
 /* ---------------- continuous-batching server (no reference counterpart) ----------------
  * Requests are lists of device-resident polynomials (ntt_c_desc, transformed in

@@ -269,6 +269,16 @@ struct PassJob {
 constexpr uint32_t kLogNT = 9;
 constexpr uint32_t kNT = 1u << kLogNT;
 
+// This thread's index, opaque to the optimiser: the per-lane offsets each
+// phase of a pass derives from it are recomputed where the phase runs instead
+// of being hoisted out of a tile loop (pipe_kernel) and held in registers
+// across the whole loop.
+__device__ __forceinline__ uint32_t ltid() {
+    uint32_t t = threadIdx.x;
+    __asm__ volatile("" : "+v"(t));
+    return t;
+}
+
 // LDS addressing by byte offset.  The pass kernel declares no static LDS, so
 // its dynamic LDS starts at LDS address 0 (launch_pass_kernel checks the
 // kernel's static size once) and a byte offset is the address itself.  Built
@@ -422,7 +432,7 @@ __device__ __forceinline__ void ct_round(ZP z, const M& mod, uint32_t l0, uint32
 #pragma unroll
     for (int v = 0; v < (1 << m); v++) vsl[v] = swz((uint32_t)v << (log_t + log_c)) * VB;
     const uint32_t lowmask = (1u << (log_t + log_c)) - 1u;
-    for (uint32_t u = threadIdx.x; u < units; u += kNT) {
+    for (uint32_t u = ltid(); u < units; u += kNT) {
         // row-major word of (base, c): u's low log_t + log_c bits stay, i0 moves up by m
         const uint32_t word = (u & lowmask) | ((u & ~lowmask) << m);
         const uint32_t i0 = u >> (log_t + log_c);
@@ -535,7 +545,7 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint3
     const uint32_t R = 1u << log_r, C = 1u << log_c;
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
-    const uint32_t T = kNT, tid = threadIdx.x;
+    const uint32_t T = kNT, tid = ltid();
     const bool cf = J.flags & JF_STORE_CFAST;
     const uint32_t sweep = PER * T;
     const TwP<V> scale{(V)J.scale_w, (V)J.scale_wp};
@@ -659,7 +669,7 @@ __device__ __forceinline__ void store_loop_step(const PassJob& J, const Mod32& m
                                                 TP tl, TP tm, TP th) {
     using V = uint32_t;
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = ltid();
     const uint32_t log_tpc = kLogNT - log_c;  // 2^log_tpc threads per column
     const uint32_t cl = tid >> log_tpc, m = tid & ((1u << log_tpc) - 1u);
     const uint32_t cg = col0 + cl;
@@ -755,7 +765,7 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
     const uint32_t R = 1u << log_r, C = 1u << log_c;
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
-    const uint32_t T = kNT, tid = threadIdx.x;
+    const uint32_t T = kNT, tid = ltid();
     const bool cf = J.flags & JF_LOAD_CFAST;
     const uint32_t sweep = PER * T;
     const uint32_t sh_sr = J.sh_sr, sh_sc = J.sh_sc;
@@ -854,7 +864,7 @@ __device__ __forceinline__ void stage_tables(const PassJob& J, const Staged<type
     using V = typename M::V;
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r;
     const uint32_t R = 1u << log_r;
-    const uint32_t T = kNT, tid = threadIdx.x;
+    const uint32_t T = kNT, tid = ltid();
     const uint32_t flags = J.flags;
     if (!row_only && (flags & JF_Z_LDS)) {
         constexpr bool mf = MF && std::is_same<M, Mod32>::value;
@@ -964,6 +974,121 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
     }
     lds_barrier();
     ct_and_store<M, Sout, MAXM, P4, LR, LC, MF>(J, t, poly, col0, full, mat);
+}
+
+// ---- one pass over many tiles per workgroup, the next tile's loads in flight ----
+//
+// pipe_kernel runs a single job's full, shape-specialised tiles (2^LR rows x
+// 2^LC columns, vector loads and stores, tables in LDS).  Each workgroup walks
+// a range of tiles and issues the global loads of its next tile into
+// registers (tile_issue: 2^(LR+LC) / (4 * threads) 16-byte loads per thread)
+// before it transforms and stores the current one, then writes them to LDS
+// (tile_commit) once the current tile's store has read the buffer.  The
+// rounds and the store touch only LDS (JF_Z_LDS, post tables in LDS) and
+// issue no global loads the compiler would have to wait behind the prefetch
+// (the first pass's stepped store reads its column table once per column).
+template <int LR, int LC, class Sin>
+struct TileLoad {
+    static constexpr uint32_t kN = (1u << (LR + LC)) / (4u * kNT);
+    static_assert(kN >= 1, "tile smaller than one vector load per thread");
+    Sin v[kN][4];
+};
+
+template <int LR, int LC, class Sin>
+__device__ __forceinline__ void tile_issue(const PassJob& J, uint32_t poly, uint32_t col0, TileLoad<LR, LC, Sin>& L) {
+    constexpr uint32_t R = 1u << LR, C = 1u << LC, sweep = 4u * kNT;
+    const bool cf = J.flags & JF_LOAD_CFAST;
+    const uint32_t sh_sr = J.sh_sr, sh_sc = J.sh_sc;
+    auto rc = [&](uint32_t i, uint32_t& r, uint32_t& c) {
+        if (cf) { r = i >> LC; c = i & (C - 1u); }
+        else { c = i >> LR; r = i & (R - 1u); }
+    };
+    uint32_t lr, lc;
+    rc(ltid() * 4u, lr, lc);
+    const uint32_t l_off = (lr << sh_sr) + (lc << sh_sc);
+    const Sin* __restrict__ in =
+        reinterpret_cast<const Sin*>(J.in) + (int64_t)poly * J.in_pstride + ((int64_t)col0 << sh_sc);
+#pragma unroll
+    for (uint32_t u = 0; u < TileLoad<LR, LC, Sin>::kN; u++) {
+        uint32_t ur, uc;
+        rc(u * sweep, ur, uc);
+        load4<Sin>(in + (((ur << sh_sr) + (uc << sh_sc)) + l_off), L.v[u]);
+    }
+}
+
+template <int LR, int LC, class Sin, class M>
+__device__ __forceinline__ void tile_commit(const PassJob& J, const TileLoad<LR, LC, Sin>& L) {
+    using V = typename M::V;
+    constexpr uint32_t R = 1u << LR, C = 1u << LC, sweep = 4u * kNT, VB = sizeof(V);
+    const bool cf = J.flags & JF_LOAD_CFAST;
+    auto rc = [&](uint32_t i, uint32_t& r, uint32_t& c) {
+        if (cf) { r = i >> LC; c = i & (C - 1u); }
+        else { c = i >> LR; r = i & (R - 1u); }
+    };
+    auto slot = [&](uint32_t r, uint32_t c) -> uint32_t { return swz((r << LC) | c) * VB; };
+    uint32_t lr, lc;
+    rc(ltid() * 4u, lr, lc);
+    const uint32_t l_slot = slot(lr, lc);
+    uint32_t e_x[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) e_x[e] = cf ? (uint32_t)e * VB : swz((uint32_t)e << LC) * VB;
+#pragma unroll
+    for (uint32_t u = 0; u < TileLoad<LR, LC, Sin>::kN; u++) {
+        uint32_t ur, uc;
+        rc(u * sweep, ur, uc);
+        const uint32_t s = l_slot ^ slot(ur, uc);
+#pragma unroll
+        for (int e = 0; e < 4; e++) lds_st<V>(s ^ e_x[e], (V)L.v[u][e]);
+    }
+}
+
+// Tiles are split into 8 contiguous ranges, range x walked by the workgroups
+// b = x mod 8 (so that a range's tiles mostly share one XCD's L2, as in
+// xcd_remap); the host launches a multiple of 8 workgroups on a tile count
+// divisible by 8.  At most 6 waves per SIMD: the prefetch registers must not
+// cost the third workgroup per CU.
+template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC, bool MF>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) pipe_kernel(PassJob J) {
+    using V = typename M::V;
+    const uint32_t T = J.tiles_per_poly * (J.npoly ? J.npoly : 1u);
+    const uint32_t x = blockIdx.x & 7u, per = T >> 3, gs = gridDim.x >> 3;
+    uint32_t i = blockIdx.x >> 3;
+    if (i >= per) return;
+    const uint32_t tpp = J.tiles_per_poly;
+    auto tile_of = [&](uint32_t k, uint32_t& poly, uint32_t& col0) {
+        const uint32_t g = x * per + k;
+        poly = g / tpp;
+        col0 = (g - poly * tpp) << LC;
+    };
+    const Staged<V> t = staged_of<M, P4, LR, MF>(J);
+    MfMat mat{};
+    TileLoad<LR, LC, Sin> L;
+    uint32_t poly, col0;
+    tile_of(i, poly, col0);
+    tile_issue<LR, LC, Sin>(J, poly, col0, L);
+    stage_tables<M, P4, LR, MF>(J, t, col0, false);
+    if constexpr (MF && std::is_same<M, Mod32>::value) mf_load_mat(J.mz, mat);
+    tile_commit<LR, LC, Sin, M>(J, L);
+    lds_barrier();
+    const bool row_post = P4 && (J.flags & JF_POST_ROW);
+    for (;;) {
+        const uint32_t ni = i + gs;
+        const bool more = ni < per;
+        uint32_t npl = 0, ncol0 = 0;
+        if (more) {
+            tile_of(ni, npl, ncol0);
+            tile_issue<LR, LC, Sin>(J, npl, ncol0, L);
+        }
+        ct_and_store<M, Sout, MAXM, P4, LR, LC, MF>(J, t, poly, col0, true, mat);
+        if (!more) break;
+        lds_barrier();  // the store has read the tile (and the row table)
+        if (row_post && (ncol0 >> J.row_shift) != (col0 >> J.row_shift)) stage_tables<M, P4, LR, MF>(J, t, ncol0, true);
+        tile_commit<LR, LC, Sin, M>(J, L);
+        lds_barrier();
+        i = ni;
+        poly = npl;
+        col0 = ncol0;
+    }
 }
 
 // Workgroup index -> logical tile so that consecutive tiles run on the same

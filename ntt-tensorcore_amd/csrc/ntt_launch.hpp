@@ -55,6 +55,43 @@ hipError_t launch_queue_kernel(const PassJob* djobs, const uint32_t* tjob, const
     return hipGetLastError();
 }
 
+// Launches pipe_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC, MF> (one
+// job, several tiles per workgroup with the next tile's loads in flight).
+template <class M, class Sin, class Sout, bool P4, int LR, int LC, bool MF>
+hipError_t launch_pipe_kernel(const PassJob& J, uint32_t nblocks, uint32_t lds, hipStream_t stream) {
+    auto fn = pipe_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC, MF>;
+    static std::once_flag once;
+    static hipError_t ready = hipSuccess;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPassLdsMax);
+        hipFuncAttributes at{};
+        ready = hipFuncGetAttributes(&at, (const void*)fn);
+        if (ready == hipSuccess && at.sharedSizeBytes != 0) ready = hipErrorInvalidKernelFile;
+    });
+    if (ready != hipSuccess) return ready;
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(kPassThreads), lds, stream, J);
+    return hipGetLastError();
+}
+
+// Workgroups of pipe_kernel<...> that fit on one CU with `lds` bytes of LDS.
+template <class M, class Sin, class Sout, bool P4, int LR, int LC, bool MF>
+int pipe_kernel_per_cu(uint32_t lds) {
+    auto fn = pipe_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC, MF>;
+    (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPassLdsMax);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kPassThreads, lds) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
+}
+
+// Tile shapes with a pipelined instance (32-bit coefficients, matrix-core rounds).
+#define NTT_PIPE_SHAPES(X) X(8, 4) X(8, 5)
+#define NTT_PIPE_INSTANCE_1(EXT, LR, LC, P4)                                                                   \
+    EXT template hipError_t launch_pipe_kernel<Mod32, uint32_t, uint32_t, P4, LR, LC, true>(                    \
+        const PassJob&, uint32_t, uint32_t, hipStream_t);                                                       \
+    EXT template int pipe_kernel_per_cu<Mod32, uint32_t, uint32_t, P4, LR, LC, true>(uint32_t);
+#define NTT_PIPE_INSTANCE_EXTERN(LR, LC) NTT_PIPE_INSTANCE_1(extern, LR, LC, false) NTT_PIPE_INSTANCE_1(extern, LR, LC, true)
+#define NTT_PIPE_INSTANCE(LR, LC) NTT_PIPE_INSTANCE_1(, LR, LC, false) NTT_PIPE_INSTANCE_1(, LR, LC, true)
+
 // Workgroups of queue_kernel<...> that fit on one CU with `lds` bytes of LDS.
 template <class M, class Sin, class Sout, bool P4, bool MF>
 int queue_kernel_per_cu(uint32_t lds) {
@@ -132,5 +169,6 @@ NTT_PASS_INSTANCES(extern, ModGold, uint64_t, uint64_t)
 NTT_SHAPES(NTT_SHAPE_INSTANCE_EXTERN)
 NTT_SHAPES_U64(NTT_SHAPE_U64_INSTANCE_EXTERN)
 NTT_QUEUE_INSTANCES(extern)
+NTT_PIPE_SHAPES(NTT_PIPE_INSTANCE_EXTERN)
 
 }  // namespace nttk

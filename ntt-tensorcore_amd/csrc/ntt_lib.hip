@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -897,6 +898,72 @@ static uint32_t group_shape(const PassJob* jobs, size_t n, Arith ar, Store sin, 
     return sh;
 }
 
+// NTT_OPT_PIPELINE (ntt_c_set_option); on by default: 1.7 % on the 2^24
+// transform, same box, alternating runs.
+static std::atomic<int64_t> g_opt_pipeline{1};
+
+// CUs of the current device (cached per device).
+static uint32_t device_cus() {
+    static std::mutex mu;
+    static std::map<int, uint32_t> known;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(dev);
+    if (it != known.end()) return it->second;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    known[dev] = (uint32_t)cus;
+    return (uint32_t)cus;
+}
+
+// The pipelined kernel for one pass job, when enabled and the job has a
+// pipelined instance: 32-bit storage and arithmetic, matrix-core rounds, a
+// shape of NTT_PIPE_SHAPES, a tile count divisible by 8 and at least two tiles
+// per workgroup slot.  Returns false to fall back to one tile per workgroup.
+static bool launch_pipe(const PassJob& J, Arith ar, Store sin, Store sout, uint32_t shape, uint32_t lds,
+                        hipStream_t stream, ntt_status* st) {
+    if (g_opt_pipeline.load() == 0 || ar != Arith::M32 || sin != Store::U32 || sout != Store::U32 ||
+        !(J.flags & JF_MFMA) || shape == 0)
+        return false;
+    const uint32_t tiles = job_tiles(J), cus = device_cus();
+    if (tiles % 8 != 0 || cus == 0) return false;
+    const bool p4 = J.flags & JF_POST4;
+    int per_cu = 0;
+    switch (shape) {
+#define NTT_PIPE_CASE(LR, LC)                                                                                    \
+    case (LR << 8) | LC:                                                                                          \
+        per_cu = p4 ? pipe_kernel_per_cu<Mod32, uint32_t, uint32_t, true, LR, LC, true>(lds)                     \
+                    : pipe_kernel_per_cu<Mod32, uint32_t, uint32_t, false, LR, LC, true>(lds);                   \
+        break;
+        NTT_PIPE_SHAPES(NTT_PIPE_CASE)
+#undef NTT_PIPE_CASE
+        default: return false;
+    }
+    uint32_t nb = std::min<uint32_t>(tiles / 2, (uint32_t)per_cu * cus) & ~7u;
+    if (nb < 8) return false;
+    hipError_t e = hipSuccess;
+    switch (shape) {
+#define NTT_PIPE_CASE(LR, LC)                                                                                    \
+    case (LR << 8) | LC:                                                                                          \
+        e = p4 ? launch_pipe_kernel<Mod32, uint32_t, uint32_t, true, LR, LC, true>(J, nb, lds, stream)           \
+               : launch_pipe_kernel<Mod32, uint32_t, uint32_t, false, LR, LC, true>(J, nb, lds, stream);         \
+        break;
+        NTT_PIPE_SHAPES(NTT_PIPE_CASE)
+#undef NTT_PIPE_CASE
+        default: return false;
+    }
+    *st = e == hipSuccess ? NTT_OK : fail(NTT_ERR_DEVICE, std::string("HIP: ") + hipGetErrorString(e));
+    return true;
+}
+
+extern "C" ntt_status ntt_c_set_option(uint32_t option, int64_t value, int64_t* old) {
+    if (option != NTT_OPT_PIPELINE) return fail(NTT_ERR_INVALID_ARG, "set_option: unknown option");
+    const int64_t prev = g_opt_pipeline.exchange(value ? 1 : 0);
+    if (old) *old = prev;
+    return NTT_OK;
+}
+
 // Runs plan P on npoly polynomials at data (stride n, or dstride); with
 // mul_a, on mul_a*mul_b mod q elementwise, the result going to data
 // (make_jobs).
@@ -906,9 +973,12 @@ static ntt_status run_plan(const Plan& P, void* data, Store st, uint64_t npoly, 
     Store io[4][2];
     make_jobs(P, data, st, npoly, work, jobs, io, mul_a, mul_b, dstride);
     for (int i = 0; i < P.npass; i++) {
-        ntt_status s = launch_pass(P.arith, io[i][0], io[i][1], job_kind(jobs[i]), nullptr, 0, jobs[i],
-                                   job_tiles(jobs[i]), job_lds(jobs[i], P.arith), stream,
-                                   pass_shape_of(jobs[i], P.arith, io[i][0], io[i][1]));
+        const uint32_t shape = pass_shape_of(jobs[i], P.arith, io[i][0], io[i][1]);
+        const uint32_t lds = job_lds(jobs[i], P.arith);
+        ntt_status s = NTT_OK;
+        if (!launch_pipe(jobs[i], P.arith, io[i][0], io[i][1], shape, lds, stream, &s))
+            s = launch_pass(P.arith, io[i][0], io[i][1], job_kind(jobs[i]), nullptr, 0, jobs[i], job_tiles(jobs[i]),
+                            lds, stream, shape);
         if (s != NTT_OK) return s;
     }
     return NTT_OK;

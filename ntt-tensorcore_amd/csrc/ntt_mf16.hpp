@@ -80,7 +80,7 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
                                          uint32_t log_c) {
     const uint32_t sh = log_r - l0 - 4u + log_c;  // log2 of the word stride of v within a unit
     const uint32_t ngroups = 1u << (log_r + log_c - 8u);
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = ltid();
     const uint32_t lane = tid & 63u, g = lane >> 4, n = lane & 15u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t lowmask = (1u << sh) - 1u;

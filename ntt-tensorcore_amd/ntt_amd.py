@@ -98,6 +98,7 @@ def lib():
         "ntt_c_batch_launch": [vp, vp],
         "ntt_c_batch_launch_graph": [vp, vp],
         "ntt_c_batch_set_queue": [vp, i32, u32],
+        "ntt_c_set_option": [u32, ctypes.c_int64, P(ctypes.c_int64)],
         "ntt_c_batch_check": [vp, vp],
         "ntt_c_batch_destroy": [vp],
         "ntt_c_batch_info": [vp, P(u32), P(u64), P(u64)],
@@ -154,6 +155,16 @@ def register_sigs(fn):
     _EXTRA_SIGS.append(fn)
     if _lib is not None:
         fn(_lib)
+
+
+OPT_PIPELINE = 1
+
+
+def set_option(option, value):
+    """Process-wide kernel choice (ntt_c_set_option); returns the old value."""
+    old = ctypes.c_int64()
+    check(lib().ntt_c_set_option(option, value, ctypes.byref(old)))
+    return old.value
 
 
 def check(status):
