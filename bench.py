@@ -88,15 +88,15 @@ def headline(args, ctx, stream):
     ms_step = ms / args.steps
     bfly = 2 * W.butterflies(log_n)
     t_transform_s = ms_step / 2 / 1e3
-    # The dominant kernel is the pass kernel: a transform is P back-to-back
-    # launches of it, each moving one pass over the coefficients.  One launch
+    # The dominant kernel is the (pipelined) pass kernel: a transform is P
+    # back-to-back launches of it, each moving one pass over the coefficients.  One launch
     # processes 1/P of a transform's algorithmic bytes (2*n*4), so bytes per
     # launch / average launch time = 2*n*4 / transform time.
     npass = len(plan["split"])
     launch_s = t_transform_s / npass
     achieved = (2 * n * 4 / npass) / launch_s / 1e9
     traffic = None
-    tf = os.path.join(REPO, "profiles", f"pass_kernel_negacyclic_bb_2e{log_n}_traffic.json")
+    tf = os.path.join(REPO, "profiles", f"pipe_kernel_negacyclic_bb_2e{log_n}_traffic.json")
     if os.path.exists(tf):
         t = json.load(open(tf))
         if t.get("log_n") == log_n and t.get("split") == plan["split"] and t.get("log_e") == plan["log_e"] \
