@@ -87,16 +87,16 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
     // row-major word of unit u's element 0: u's low sh bits stay, the rest moves up by 4
     auto word = [&](uint32_t u) -> uint32_t { return (u & lowmask) | ((u & ~lowmask) << 4); };
     // Lane (g, n) of group ub reads unit ub | n, elements v = 4g + e.  Every map
-    // here is GF(2)-linear and ub, n occupy disjoint bits, so an address is
-    // (lane part, kept in registers for the round) ^ (group part, a scalar).
-    // (The empty asm keeps the compiler from refactoring la ^ U back into
-    // (word ^ word) * 4, one more shift per access.)
-    uint32_t la[4];
+    // here is GF(2)-linear and ub, n, g, e occupy disjoint bits, so an address
+    // is (lane part la, one register for the round) ^ (element part se[e], a
+    // scalar) ^ (group part U, a scalar): one vector XOR per access.
+    // (The empty asm keeps the compiler from refactoring la ^ (se ^ U) back
+    // into (word ^ word) * 4, more vector ops per access.)
+    uint32_t la = (swz(word(n)) ^ swz((4u * g) << sh)) * 4u;
+    __asm__("" : "+v"(la));
+    uint32_t se[4];
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-        la[e] = swz(word(n) | ((4u * g + (uint32_t)e) << sh)) * 4u;
-        __asm__("" : "+v"(la[e]));
-    }
+    for (int e = 0; e < 4; e++) se[e] = __builtin_amdgcn_readfirstlane(swz((uint32_t)e << sh) * 4u);
     // twiddle index (((ub | n) >> sh) << 4) | 4g = lane part | (ub >> sh) << 4
     const uint32_t lti = ((n >> sh) << 4) | (4u * g);
     // with the table in LDS: byte address of the lane part
@@ -114,7 +114,7 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
         const uint32_t U = __builtin_amdgcn_readfirstlane(swz(word(ub)) * 4u);
         uint32_t x[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) x[e] = lds_ld<uint32_t>(la[e] ^ U);
+        for (int e = 0; e < 4; e++) x[e] = lds_ld<uint32_t>(la ^ (se[e] ^ U));
         if (TW) {
             const uint32_t tu = __builtin_amdgcn_readfirstlane((ub >> sh) << 4);
             TwP<uint32_t> t[4];
@@ -148,7 +148,7 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
                 y = y - __umulhi(y, qr) * q;  // [0, 2q)
                 y = min(y, y - q);
             }
-            lds_st<uint32_t>(la[e] ^ U, y);
+            lds_st<uint32_t>(la ^ (se[e] ^ U), y);
         }
     }
 }
