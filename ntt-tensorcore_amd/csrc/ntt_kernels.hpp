@@ -432,7 +432,9 @@ __device__ __forceinline__ void ct_round(ZP z, const M& mod, uint32_t l0, uint32
 #pragma unroll
     for (int v = 0; v < (1 << m); v++) vsl[v] = swz((uint32_t)v << (log_t + log_c)) * VB;
     const uint32_t lowmask = (1u << (log_t + log_c)) - 1u;
-    for (uint32_t u = ltid(); u < units; u += kNT) {
+    // (threadIdx.x, not ltid(): these rounds run in the one-tile kernels only,
+    // and the 62-bit instances spill with the opaque index)
+    for (uint32_t u = threadIdx.x; u < units; u += kNT) {
         // row-major word of (base, c): u's low log_t + log_c bits stay, i0 moves up by m
         const uint32_t word = (u & lowmask) | ((u & ~lowmask) << m);
         const uint32_t i0 = u >> (log_t + log_c);
