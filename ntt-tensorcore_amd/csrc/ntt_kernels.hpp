@@ -536,9 +536,12 @@ constexpr int kUnroll = 8;
 //   SWT  tables are staged in LDS at tsw(i),
 //   LR, LC the tile shape log2 R, log2 C when fixed at compile time (LR = 0:
 //   taken from the job),
-//   LZ   (QM 3) outputs in [0, 2q) instead of canonical (JF_LAZY_OUT).
+//   LZ   (QM 3) outputs in [0, 2q) instead of canonical (JF_LAZY_OUT),
+//   CF   the store direction (JF_STORE_CFAST) when fixed at compile time (1 or
+//        0; -1: taken from the job), so that shape-specialised kernels fold
+//        the uniform part of every address into constants and scalars.
 template <int PER, int POST, int NTAB, int QM, bool PRED, bool SWT, int LR, int LC, class M, class Sout,
-          bool LZ = false, class TP>
+          bool LZ = false, int CF = -1, class TP>
 __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint32_t poly, uint32_t col0, TP tl, TP tm,
                                            TP th) {
     using V = typename M::V;
@@ -548,7 +551,7 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint3
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
     const uint32_t T = kNT, tid = ltid();
-    const bool cf = J.flags & JF_STORE_CFAST;
+    const bool cf = CF < 0 ? (J.flags & JF_STORE_CFAST) != 0 : CF != 0;
     const uint32_t sweep = PER * T;
     const TwP<V> scale{(V)J.scale_w, (V)J.scale_wp};
     const uint32_t o_lbits = J.o_lbits, omask = (1u << o_lbits) - 1u;
@@ -714,27 +717,27 @@ __device__ __forceinline__ void store_loop_step(const PassJob& J, const Mod32& m
     }
 }
 
-template <int PER, bool P4, bool SWT, int LR, int LC, class M, class Sout, class TP>
+template <int PER, bool P4, bool SWT, int LR, int LC, class M, class Sout, int CF = -1, class TP>
 __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, uint32_t poly, uint32_t col0, bool pred,
                                                TP tl, TP tm, TP th) {
     const uint32_t post = J.post;
     if (P4 && post == POST_FOURSTEP && SWT && (J.flags & JF_POST_ROW)) {
-        if (pred) store_loop<PER, 2, 2, 3, true, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
-        else if (J.flags & JF_LAZY_OUT) store_loop<PER, 2, 2, 3, false, SWT, LR, LC, M, Sout, true>(J, mod, poly, col0, tl, tm, th);
-        else store_loop<PER, 2, 2, 3, false, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
-    } else if (std::is_same<M, Mod32>::value && PER == 4 && P4 && post == POST_FOURSTEP && SWT && !pred &&
+        if (pred) store_loop<PER, 2, 2, 3, true, SWT, LR, LC, M, Sout, false, CF>(J, mod, poly, col0, tl, tm, th);
+        else if (J.flags & JF_LAZY_OUT) store_loop<PER, 2, 2, 3, false, SWT, LR, LC, M, Sout, true, CF>(J, mod, poly, col0, tl, tm, th);
+        else store_loop<PER, 2, 2, 3, false, SWT, LR, LC, M, Sout, false, CF>(J, mod, poly, col0, tl, tm, th);
+    } else if (std::is_same<M, Mod32>::value && PER == 4 && P4 && post == POST_FOURSTEP && SWT && !pred && CF != 1 &&
                (J.flags & JF_POST_STEP)) {
         if constexpr (std::is_same<M, Mod32>::value && PER == 4) {
             if (J.mid_bits) store_loop_step<3, SWT, LR, LC, Sout>(J, mod, poly, col0, tl, tm, th);
             else store_loop_step<2, SWT, LR, LC, Sout>(J, mod, poly, col0, tl, tm, th);
         }
     } else if (P4 && post == POST_FOURSTEP) {
-        const bool cf = J.flags & JF_STORE_CFAST;
+        const bool cf = CF < 0 ? (J.flags & JF_STORE_CFAST) != 0 : CF != 0;
         const int qm = PER == 4 ? (cf ? (J.e_shift >= 2 ? 1 : 0) : 2) : 0;
         const bool three = J.mid_bits != 0;
-#define NTT_STORE4(NT, QM)                                                                          \
-    (pred ? store_loop<PER, 2, NT, QM, true, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th) \
-          : store_loop<PER, 2, NT, QM, false, SWT, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th))
+#define NTT_STORE4(NT, QM)                                                                                      \
+    (pred ? store_loop<PER, 2, NT, QM, true, SWT, LR, LC, M, Sout, false, CF>(J, mod, poly, col0, tl, tm, th) \
+          : store_loop<PER, 2, NT, QM, false, SWT, LR, LC, M, Sout, false, CF>(J, mod, poly, col0, tl, tm, th))
         if (three) {
             if (qm == 1) NTT_STORE4(3, 1);
             else if (qm == 2) NTT_STORE4(3, 2);
@@ -746,11 +749,11 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
         }
 #undef NTT_STORE4
     } else if (post == POST_SCALE) {
-        if (pred) store_loop<PER, 1, 2, 0, true, false, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
-        else store_loop<PER, 1, 2, 0, false, false, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
+        if (pred) store_loop<PER, 1, 2, 0, true, false, LR, LC, M, Sout, false, CF>(J, mod, poly, col0, tl, tm, th);
+        else store_loop<PER, 1, 2, 0, false, false, LR, LC, M, Sout, false, CF>(J, mod, poly, col0, tl, tm, th);
     } else {
-        if (pred) store_loop<PER, 0, 2, 0, true, false, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
-        else store_loop<PER, 0, 2, 0, false, false, LR, LC, M, Sout>(J, mod, poly, col0, tl, tm, th);
+        if (pred) store_loop<PER, 0, 2, 0, true, false, LR, LC, M, Sout, false, CF>(J, mod, poly, col0, tl, tm, th);
+        else store_loop<PER, 0, 2, 0, false, false, LR, LC, M, Sout, false, CF>(J, mod, poly, col0, tl, tm, th);
     }
 }
 
@@ -759,7 +762,7 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
 // stores in[i]*in2[i] mod q (JF_MUL_IN).
 //   pre() runs once, after the first batch's global loads are issued and before
 //   their LDS stores, so that loads it issues share their latency.
-template <int PER, bool PRED, int LR, int LC, class Sin, class M, bool MUL = false, class Pre>
+template <int PER, bool PRED, int LR, int LC, class Sin, class M, bool MUL = false, int CF = -1, class Pre>
 __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint32_t col0, Pre pre) {
     using V = typename M::V;
     constexpr uint32_t VB = sizeof(V);
@@ -768,7 +771,7 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
     const uint32_t total = R << log_c;
     const uint32_t ncols = J.ncols;
     const uint32_t T = kNT, tid = ltid();
-    const bool cf = J.flags & JF_LOAD_CFAST;
+    const bool cf = CF < 0 ? (J.flags & JF_LOAD_CFAST) != 0 : CF != 0;
     const uint32_t sweep = PER * T;
     const uint32_t sh_sr = J.sh_sr, sh_sc = J.sh_sc;
     auto rc = [&](uint32_t i, uint32_t& r, uint32_t& c) {
@@ -920,8 +923,9 @@ __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<type
     const TwP<V>* tl = reinterpret_cast<const TwP<V>*>(J.tw_lo);
     const TwP<V>* tm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
     const TwP<V>* th = reinterpret_cast<const TwP<V>*>(J.tw_hi);
-    if (LR) {  // shape-specialised: vector stores, full tiles, LDS post tables
-        store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, false, t.pl, t.pm, t.ph);
+    if (LR) {  // shape-specialised: vector stores, full tiles, LDS post tables, fixed direction
+        if (flags & JF_STORE_CFAST) store_dispatch<4, P4, true, LR, LC, M, Sout, 1>(J, mod, poly, col0, false, t.pl, t.pm, t.ph);
+        else store_dispatch<4, P4, true, LR, LC, M, Sout, 0>(J, mod, poly, col0, false, t.pl, t.pm, t.ph);
     } else if (P4 && (flags & JF_POST_LDS)) {
         if (flags & JF_VEC4_ST) store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, t.pl, t.pm, t.ph);
         else store_dispatch<1, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, t.pl, t.pm, t.ph);
@@ -962,7 +966,8 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
             if (flags & JF_MUL_IN) load_loop<4, false, LR, LC, Sin, M, true>(J, poly, col0, pre);
             else load_loop<4, false, LR, LC, Sin, M>(J, poly, col0, pre);
         } else {
-            load_loop<4, false, LR, LC, Sin, M>(J, poly, col0, pre);
+            if (flags & JF_LOAD_CFAST) load_loop<4, false, LR, LC, Sin, M, false, 1>(J, poly, col0, pre);
+            else load_loop<4, false, LR, LC, Sin, M, false, 0>(J, poly, col0, pre);
         }
     } else if (flags & JF_MUL_IN) {  // pointwise product fused into the load (predicated: rare path)
         if (flags & JF_VEC4) load_loop<4, true, LR, LC, Sin, M, true>(J, poly, col0, pre);
@@ -996,10 +1001,10 @@ struct TileLoad {
     Sin v[kN][4];
 };
 
-template <int LR, int LC, class Sin>
+template <int LR, int LC, class Sin, bool CF>
 __device__ __forceinline__ void tile_issue(const PassJob& J, uint32_t poly, uint32_t col0, TileLoad<LR, LC, Sin>& L) {
     constexpr uint32_t R = 1u << LR, C = 1u << LC, sweep = 4u * kNT;
-    const bool cf = J.flags & JF_LOAD_CFAST;
+    constexpr bool cf = CF;
     const uint32_t sh_sr = J.sh_sr, sh_sc = J.sh_sc;
     auto rc = [&](uint32_t i, uint32_t& r, uint32_t& c) {
         if (cf) { r = i >> LC; c = i & (C - 1u); }
@@ -1018,11 +1023,11 @@ __device__ __forceinline__ void tile_issue(const PassJob& J, uint32_t poly, uint
     }
 }
 
-template <int LR, int LC, class Sin, class M>
+template <int LR, int LC, class Sin, class M, bool CF>
 __device__ __forceinline__ void tile_commit(const PassJob& J, const TileLoad<LR, LC, Sin>& L) {
     using V = typename M::V;
     constexpr uint32_t R = 1u << LR, C = 1u << LC, sweep = 4u * kNT, VB = sizeof(V);
-    const bool cf = J.flags & JF_LOAD_CFAST;
+    constexpr bool cf = CF;
     auto rc = [&](uint32_t i, uint32_t& r, uint32_t& c) {
         if (cf) { r = i >> LC; c = i & (C - 1u); }
         else { c = i >> LR; r = i & (R - 1u); }
@@ -1049,8 +1054,8 @@ __device__ __forceinline__ void tile_commit(const PassJob& J, const TileLoad<LR,
 // xcd_remap); the host launches a multiple of 8 workgroups on a tile count
 // divisible by 8.  At most 6 waves per SIMD: the prefetch registers must not
 // cost the third workgroup per CU.
-template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC, bool MF>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) pipe_kernel(PassJob J) {
+template <class M, class Sin, class Sout, int MAXM, bool P4, int LR, int LC, bool MF, bool LCF>
+__device__ __forceinline__ void pipe_run(const PassJob& J) {
     using V = typename M::V;
     const uint32_t T = J.tiles_per_poly * (J.npoly ? J.npoly : 1u);
     const uint32_t x = blockIdx.x & 7u, per = T >> 3, gs = gridDim.x >> 3;
@@ -1067,10 +1072,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) pi
     TileLoad<LR, LC, Sin> L;
     uint32_t poly, col0;
     tile_of(i, poly, col0);
-    tile_issue<LR, LC, Sin>(J, poly, col0, L);
+    tile_issue<LR, LC, Sin, LCF>(J, poly, col0, L);
     stage_tables<M, P4, LR, MF>(J, t, col0, false);
     if constexpr (MF && std::is_same<M, Mod32>::value) mf_load_mat(J.mz, mat);
-    tile_commit<LR, LC, Sin, M>(J, L);
+    tile_commit<LR, LC, Sin, M, LCF>(J, L);
     lds_barrier();
     const bool row_post = P4 && (J.flags & JF_POST_ROW);
     for (;;) {
@@ -1079,18 +1084,24 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) pi
         uint32_t npl = 0, ncol0 = 0;
         if (more) {
             tile_of(ni, npl, ncol0);
-            tile_issue<LR, LC, Sin>(J, npl, ncol0, L);
+            tile_issue<LR, LC, Sin, LCF>(J, npl, ncol0, L);
         }
         ct_and_store<M, Sout, MAXM, P4, LR, LC, MF>(J, t, poly, col0, true, mat);
         if (!more) break;
         lds_barrier();  // the store has read the tile (and the row table)
         if (row_post && (ncol0 >> J.row_shift) != (col0 >> J.row_shift)) stage_tables<M, P4, LR, MF>(J, t, ncol0, true);
-        tile_commit<LR, LC, Sin, M>(J, L);
+        tile_commit<LR, LC, Sin, M, LCF>(J, L);
         lds_barrier();
         i = ni;
         poly = npl;
         col0 = ncol0;
     }
+}
+
+template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC, bool MF>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) pipe_kernel(PassJob J) {
+    if (J.flags & JF_LOAD_CFAST) pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, true>(J);
+    else pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, false>(J);
 }
 
 // Workgroup index -> logical tile so that consecutive tiles run on the same
