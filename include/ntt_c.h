@@ -275,11 +275,14 @@ ntt_status ntt_c_autotune_clear(void);
  * out = {npass, log_r[0..3], log_e, mfma_mask}. */
 ntt_status ntt_c_split_for(uint64_t q, uint32_t log_n, uint64_t elems_in_flight, uint32_t out[7]);
 /* Process-wide kernel choice (no reference counterpart).  NTT_OPT_PIPELINE:
- * value 1 runs each pass of a single transform (32-bit storage, matrix-core
- * rounds, a pipelined tile shape) as workgroups that walk several tiles and
- * load the next tile while transforming the current one (the default); 0 runs
- * one tile per workgroup.  Returns the previous value in *old (may be NULL).
- * Results are identical either way. */
+ * value 1 (the default) runs a pass (32-bit storage, matrix-core rounds, a
+ * pipelined tile shape; a single transform, or the polynomials of a batch
+ * group that share one plan and sit at equal spacing) as workgroups that walk
+ * several tiles and load the next tile while transforming the current one,
+ * where the pass has at least two tiles per workgroup slot of the device;
+ * 2 does so for every such pass with at least 16 tiles (testing); 0 runs one
+ * tile per workgroup.  Other values are refused.  Returns the previous value
+ * in *old (may be NULL).  Results are identical either way. */
 #define NTT_OPT_PIPELINE 1
 ntt_status ntt_c_set_option(uint32_t option, int64_t value, int64_t* old);
 

@@ -1076,6 +1076,11 @@ __device__ __forceinline__ void pipe_run(const PassJob& J) {
     stage_tables<M, P4, LR, MF>(J, t, col0, false);
     if constexpr (MF && std::is_same<M, Mod32>::value) mf_load_mat(J.mz, mat);
     tile_commit<LR, LC, Sin, M, LCF>(J, L);
+    // Every load before the loop (matrix fragments, tables) has landed.  The
+    // loop's first use of the fragments would otherwise carry a wait of its
+    // own, counted past the next tile's loads issued just before it, and the
+    // first round of every tile would stall until the prefetch arrived.
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     lds_barrier();
     const bool row_post = P4 && (J.flags & JF_POST_ROW);
     for (;;) {
@@ -1098,8 +1103,14 @@ __device__ __forceinline__ void pipe_run(const PassJob& J) {
     }
 }
 
+// Tiles of 2^14 words (64 KiB) leave room for two workgroups per CU (4 waves
+// per SIMD), so their instances may hold the 32 registers of the next tile's
+// loads within 128 VGPRs; smaller tiles run three per CU (6 waves, 80 VGPRs).
+template <int LR, int LC>
+constexpr int pipe_waves() { return LR + LC >= 14 ? 4 : 6; }
+
 template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC, bool MF>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) pipe_kernel(PassJob J) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pipe_waves<LR, LC>()))) pipe_kernel(PassJob J) {
     if (J.flags & JF_LOAD_CFAST) pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, true>(J);
     else pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, false>(J);
 }

@@ -919,12 +919,16 @@ static uint32_t device_cus() {
 
 // The pipelined kernel for one pass job, when enabled and the job has a
 // pipelined instance: 32-bit storage and arithmetic, matrix-core rounds, a
-// shape of NTT_PIPE_SHAPES, a tile count divisible by 8 and at least two tiles
-// per workgroup slot.  Returns false to fall back to one tile per workgroup.
+// shape of NTT_PIPE_SHAPES and a tile count divisible by 8.  By default only
+// where every workgroup slot of the device gets two tiles or more: with fewer
+// tiles the pipelined grid would leave CUs idle that one tile per workgroup
+// fills (option value 2 lifts this, down to 16 tiles).  Returns false to fall
+// back to one tile per workgroup.
 static bool launch_pipe(const PassJob& J, Arith ar, Store sin, Store sout, uint32_t shape, uint32_t lds,
                         hipStream_t stream, ntt_status* st) {
-    if (g_opt_pipeline.load() == 0 || ar != Arith::M32 || sin != Store::U32 || sout != Store::U32 ||
-        !(J.flags & JF_MFMA) || shape == 0)
+    const int64_t opt = g_opt_pipeline.load();
+    if (opt == 0 || ar != Arith::M32 || sin != Store::U32 || sout != Store::U32 || !(J.flags & JF_MFMA) ||
+        shape == 0)
         return false;
     const uint32_t tiles = job_tiles(J), cus = device_cus();
     if (tiles % 8 != 0 || cus == 0) return false;
@@ -940,7 +944,9 @@ static bool launch_pipe(const PassJob& J, Arith ar, Store sin, Store sout, uint3
 #undef NTT_PIPE_CASE
         default: return false;
     }
-    uint32_t nb = std::min<uint32_t>(tiles / 2, (uint32_t)per_cu * cus) & ~7u;
+    const uint32_t slots = (uint32_t)per_cu * cus;
+    if (opt == 1 && tiles < 2 * slots) return false;
+    uint32_t nb = std::min<uint32_t>(tiles / 2, slots) & ~7u;
     if (nb < 8) return false;
     hipError_t e = hipSuccess;
     switch (shape) {
@@ -959,7 +965,8 @@ static bool launch_pipe(const PassJob& J, Arith ar, Store sin, Store sout, uint3
 
 extern "C" ntt_status ntt_c_set_option(uint32_t option, int64_t value, int64_t* old) {
     if (option != NTT_OPT_PIPELINE) return fail(NTT_ERR_INVALID_ARG, "set_option: unknown option");
-    const int64_t prev = g_opt_pipeline.exchange(value ? 1 : 0);
+    if (value < 0 || value > 2) return fail(NTT_ERR_INVALID_ARG, "set_option: NTT_OPT_PIPELINE takes 0, 1 or 2");
+    const int64_t prev = g_opt_pipeline.exchange(value);
     if (old) *old = prev;
     return NTT_OK;
 }
@@ -1696,6 +1703,54 @@ static ntt_status build_queue(ntt_c_batch::Queue& Q, const std::vector<std::vect
     return NTT_OK;
 }
 
+// Jobs of one batch group that differ only in where their polynomials sit
+// run as one job over npoly polynomials when the polynomials are equally
+// spaced on both sides (a multiple of the element size, and of 16 bytes on a
+// side with vector accesses): a single-job group can then take the pipelined
+// kernel, and fewer jobs leave a shorter tile-to-job search.  Multi-pass jobs
+// only (s_c == 1): a one-pass job lays its polynomials out as tile columns.
+static bool same_but_place(const PassJob& a, const PassJob& b) {
+    PassJob x = a, y = b;
+    x.in = y.in = nullptr;
+    x.out = y.out = nullptr;
+    x.tile_begin = y.tile_begin = 0;
+    return memcmp(&x, &y, sizeof(PassJob)) == 0;
+}
+
+static std::vector<PassJob> merge_jobs(const std::vector<PassJob>& jobs, Store sin, Store sout) {
+    std::vector<PassJob> out;
+    const int64_t ib = sbytes(sin), ob = sbytes(sout);
+    for (size_t i = 0; i < jobs.size();) {
+        PassJob J = jobs[i];
+        size_t j = i + 1;
+        if (J.npoly == 1 && !J.in2 && J.s_c == 1 && j < jobs.size() && same_but_place(J, jobs[j])) {
+            const int64_t din = (const char*)jobs[j].in - (const char*)J.in;
+            const int64_t dout = (const char*)jobs[j].out - (const char*)J.out;
+            const bool ok = din > 0 && dout > 0 && din % ib == 0 && dout % ob == 0 &&
+                            (!(J.flags & JF_VEC4) || din % 16 == 0) && (!(J.flags & JF_VEC4_ST) || dout % 16 == 0);
+            if (ok) {
+                while (j < jobs.size() && same_but_place(J, jobs[j]) &&
+                       (const char*)jobs[j].in - (const char*)J.in == (int64_t)(j - i) * din &&
+                       (const char*)jobs[j].out - (const char*)J.out == (int64_t)(j - i) * dout)
+                    j++;
+                J.npoly = (uint32_t)(j - i);
+                J.in_pstride = din / ib;
+                J.out_pstride = dout / ob;
+            } else {
+                j = i + 1;
+            }
+        }
+        out.push_back(J);
+        i = j;
+    }
+    uint32_t begin = 0;
+    for (PassJob& J : out) {
+        J.tile_begin = begin;
+        begin += job_tiles(J);
+    }
+    return out;
+}
+
 extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count,
                                           ntt_c_batch** out) {
     if (!ctx || !out || (count && !descs)) return fail(NTT_ERR_INVALID_ARG, "batch_prepare: null argument");
@@ -1775,14 +1830,20 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
             jobs.push_back(J);
             j++;
         }
-        g.njobs = (uint32_t)jobs.size();
+        jobs = merge_jobs(jobs, g.sin, g.sout);
         g.shape = group_shape(jobs.data(), jobs.size(), g.ar, g.sin, g.sout);
         g.nblocks = blocks;
         g.lds = lds;
-        hipError_t e = hipMalloc(&g.djobs, jobs.size() * sizeof(PassJob));
-        if (e != hipSuccess) return fail(NTT_ERR_OOM, "batch_prepare: job table allocation failed");
-        e = hipMemcpy(g.djobs, jobs.data(), jobs.size() * sizeof(PassJob), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return fail(NTT_ERR_DEVICE, "batch_prepare: job table upload failed");
+        if (jobs.size() == 1) {  // one job: passed by value, no job table
+            g.inl = jobs[0];
+            g.njobs = 0;
+        } else {
+            g.njobs = (uint32_t)jobs.size();
+            hipError_t e = hipMalloc(&g.djobs, jobs.size() * sizeof(PassJob));
+            if (e != hipSuccess) return fail(NTT_ERR_OOM, "batch_prepare: job table allocation failed");
+            e = hipMemcpy(g.djobs, jobs.data(), jobs.size() * sizeof(PassJob), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(NTT_ERR_DEVICE, "batch_prepare: job table upload failed");
+        }
         B->workgroups += blocks;
         B->phases[items[i].phase].push_back(g);
         i = j;
@@ -1806,8 +1867,12 @@ static ntt_status batch_enqueue(ntt_c_batch* B, hipStream_t stream) {
     }
     for (auto& ph : B->phases)
         for (auto& g : ph) {
-            ntt_status st =
-                launch_pass(g.ar, g.sin, g.sout, g.kind, g.djobs, g.njobs, g.inl, g.nblocks, g.lds, stream, g.shape);
+            ntt_status st = NTT_OK;
+            if (g.njobs == 0 && launch_pipe(g.inl, g.ar, g.sin, g.sout, g.shape, g.lds, stream, &st)) {
+                if (st != NTT_OK) return st;
+                continue;
+            }
+            st = launch_pass(g.ar, g.sin, g.sout, g.kind, g.djobs, g.njobs, g.inl, g.nblocks, g.lds, stream, g.shape);
             if (st != NTT_OK) return st;
         }
     return NTT_OK;
