@@ -84,11 +84,11 @@ int pipe_kernel_per_cu(uint32_t lds) {
 }
 
 // Tile shapes with a pipelined instance (32-bit coefficients, matrix-core
-// rounds): the 2^12- and 2^13-word tiles of the [8, 8, 8]-style plans (A) and
-// the 2^14-word tiles of the two-pass 2^20 batch plan [11, 9] (B), one
+// rounds): tiles of 2^12 and 2^13 words (A: the [8, 8, 8]-style plans and the
+// [10, 10] two-pass plan) and of 2^14 words (B: [11, 9], [10, 10]), one
 // translation unit each.
-#define NTT_PIPE_SHAPES_A(X) X(8, 4) X(8, 5)
-#define NTT_PIPE_SHAPES_B(X) X(11, 3) X(9, 5)
+#define NTT_PIPE_SHAPES_A(X) X(8, 4) X(8, 5) X(10, 3)
+#define NTT_PIPE_SHAPES_B(X) X(11, 3) X(9, 5) X(10, 4)
 #define NTT_PIPE_SHAPES(X) NTT_PIPE_SHAPES_A(X) NTT_PIPE_SHAPES_B(X)
 #define NTT_PIPE_INSTANCE_1(EXT, LR, LC, P4)                                                                   \
     EXT template hipError_t launch_pipe_kernel<Mod32, uint32_t, uint32_t, P4, LR, LC, true>(                    \

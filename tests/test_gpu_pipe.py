@@ -39,10 +39,10 @@ def _run(nt, ctx, x, q, log_n, flags, split, log_e, mask, pipe):
 
 
 # (log n, split, log2 tile elements): every pass whose shape is (8, 4),
-# (8, 5), (9, 5) or (11, 3) takes the pipelined kernel; tile counts give 2 to
-# 16 tiles per workgroup range
+# (8, 5), (10, 3), (9, 5), (10, 4) or (11, 3) takes the pipelined kernel; tile
+# counts give 2 to 16 tiles per workgroup range
 CASES = [(16, [8, 8], 12), (16, [8, 8], 13), (17, [8, 9], 12), (20, [8, 4, 8], 12), (20, [8, 4, 8], 13),
-         (18, [8, 2, 8], 13), (18, [9, 9], 14), (20, [11, 9], 14)]
+         (18, [8, 2, 8], 13), (18, [9, 9], 14), (20, [11, 9], 14), (20, [10, 10], 13), (20, [10, 10], 14)]
 
 
 # the 30-bit prime has 2n | q - 1 only up to n = 2^18
