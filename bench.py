@@ -608,7 +608,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the NTT path runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+              "(sharding across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
     if nt.device_count() < 1:
         print("no GPU visible", file=sys.stderr)

@@ -898,8 +898,10 @@ static uint32_t group_shape(const PassJob* jobs, size_t n, Arith ar, Store sin, 
     return sh;
 }
 
-// NTT_OPT_PIPELINE (ntt_c_set_option); on by default: 1.7 % on the 2^24
-// transform, same box, alternating runs.
+// NTT_OPT_PIPELINE (ntt_c_set_option): 0 off, 1 (default) where every
+// workgroup slot gets two tiles or more, 2 wherever a pipelined instance
+// exists (16 tiles or more).  About 3 % on the 2^24 transform and 4 % on the
+// 2^20 batch of 16, measured with alternating runs on one device.
 static std::atomic<int64_t> g_opt_pipeline{1};
 
 // CUs of the current device (cached per device).

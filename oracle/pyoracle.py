@@ -244,8 +244,8 @@ def time_batch(ns, qs, dirs, threads):
 
 
 def host_threads():
-    """Worker threads for the oracle's batch transforms: the host's cores,
-    at most 16 (the GPU box's CPU share)."""
+    """Worker threads for the oracle's batch transforms: the cores this
+    process may run on, at most 16 (the CPU share a one-GPU host grants)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
