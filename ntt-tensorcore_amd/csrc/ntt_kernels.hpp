@@ -63,6 +63,11 @@ __device__ __forceinline__ uint32_t mont32(uint32_t x, uint32_t F, uint32_t q, u
     const uint32_t r = hi - __umulhi(lo * qinv, q);
     return min(r, r + q);
 }
+// The same product as a representative in (0, 2q): r lies in (-q, q).
+__device__ __forceinline__ uint32_t mont32_lazy(uint32_t x, uint32_t F, uint32_t q, uint32_t qinv) {
+    const uint32_t lo = x * F, hi = __umulhi(x, F);
+    return hi - __umulhi(lo * qinv, q) + q;
+}
 
 // q^-1 mod 2^32 for odd q: Newton's iteration from q (right to 3 bits).
 __device__ __forceinline__ uint32_t inv_mod_2_32(uint32_t q) {
@@ -220,8 +225,8 @@ enum JobFlags : uint32_t {
     JF_CT_CYC = 256u,     // CT table is cyclic (Z[2^l] = 1): round 0 skips products by 1
     JF_POST_ROW = 512u,   // post from the row table (one factor per row k), staged in LDS
     JF_POST_STEP = 1024u, // row-fast first-pass store with per-column stepped factors (tw_col)
-    JF_LAZY_OUT = 2048u,  // row-table post may leave values in [0, 2q): the next pass's matrix-core
-                          // round takes any 32-bit representative
+    JF_LAZY_OUT = 2048u,  // row-table or stepped post may leave values in [0, 2q): the next pass's
+                          // matrix-core round takes any 32-bit representative
     JF_MUL_IN = 4096u,    // the load multiplies in[i] by in2[i] mod q (pointwise product fused into
                           // the first pass of an inverse transform); inputs canonical
 };
@@ -669,7 +674,9 @@ __device__ __forceinline__ void store_loop(const PassJob& J, const M& mod, uint3
 // Wb = base^(s*kstep); rows +1..+3 first take base^(s*j) (Shoup), then every
 // element one Montgomery product by G.  Column table tw_col[c] = {base^s,
 // base^2s, base^3s, base^(s*kstep)}; 32-bit arithmetic, full tiles.
-template <int NTAB, bool SWT, int LR, int LC, class Sout, class TP>
+// LZ (JF_LAZY_OUT): outputs in (0, 2q) instead of canonical, for a next pass
+// whose first round runs on the matrix cores (any 32-bit representative).
+template <int NTAB, bool SWT, int LR, int LC, class Sout, bool LZ, class TP>
 __device__ __forceinline__ void store_loop_step(const PassJob& J, const Mod32& mod, uint32_t poly, uint32_t col0,
                                                 TP tl, TP tm, TP th) {
     using V = uint32_t;
@@ -710,7 +717,7 @@ __device__ __forceinline__ void store_loop_step(const PassJob& J, const Mod32& m
         x[2] = mod.mul_lazy(x[2], B2);
         x[3] = mod.mul_lazy(x[3], B3);
 #pragma unroll
-        for (int j = 0; j < 4; j++) x[j] = mont32(x[j], G, q, qinv);
+        for (int j = 0; j < 4; j++) x[j] = LZ ? mont32_lazy(x[j], G, q, qinv) : mont32(x[j], G, q, qinv);
         G = mod.mul(G, Wb);
         Sout v[4] = {(Sout)x[0], (Sout)x[1], (Sout)x[2], (Sout)x[3]};
         store4<Sout>(out + ((u * kstep) << J.sh_or), v);
@@ -728,8 +735,14 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
     } else if (std::is_same<M, Mod32>::value && PER == 4 && P4 && post == POST_FOURSTEP && SWT && !pred && CF != 1 &&
                (J.flags & JF_POST_STEP)) {
         if constexpr (std::is_same<M, Mod32>::value && PER == 4) {
-            if (J.mid_bits) store_loop_step<3, SWT, LR, LC, Sout>(J, mod, poly, col0, tl, tm, th);
-            else store_loop_step<2, SWT, LR, LC, Sout>(J, mod, poly, col0, tl, tm, th);
+            const bool lz = J.flags & JF_LAZY_OUT;
+            if (J.mid_bits) {
+                if (lz) store_loop_step<3, SWT, LR, LC, Sout, true>(J, mod, poly, col0, tl, tm, th);
+                else store_loop_step<3, SWT, LR, LC, Sout, false>(J, mod, poly, col0, tl, tm, th);
+            } else {
+                if (lz) store_loop_step<2, SWT, LR, LC, Sout, true>(J, mod, poly, col0, tl, tm, th);
+                else store_loop_step<2, SWT, LR, LC, Sout, false>(J, mod, poly, col0, tl, tm, th);
+            }
         }
     } else if (P4 && post == POST_FOURSTEP) {
         const bool cf = CF < 0 ? (J.flags & JF_STORE_CFAST) != 0 : CF != 0;

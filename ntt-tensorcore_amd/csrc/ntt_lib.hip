@@ -754,11 +754,11 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         src = dst;
         src_st = dst_st;
     }
-    // A row-table post whose output the next pass reads first with a
-    // matrix-core round (log_r mod 4 == 0) may leave values in [0, 2q).
+    // A row-table or stepped post whose output the next pass reads first with
+    // a matrix-core round (log_r mod 4 == 0) may leave values in [0, 2q).
     for (int i = 0; i + 1 < P.npass; i++) {
         const PassJob& Nx = out[i + 1];
-        if ((out[i].flags & JF_POST_ROW) && (Nx.flags & JF_MFMA) && (Nx.log_r & 3u) == 0 &&
+        if ((out[i].flags & (JF_POST_ROW | JF_POST_STEP)) && (Nx.flags & JF_MFMA) && (Nx.log_r & 3u) == 0 &&
             P.arith == Arith::M32 && sin_out[i][1] == Store::U32)
             out[i].flags |= JF_LAZY_OUT;
     }
