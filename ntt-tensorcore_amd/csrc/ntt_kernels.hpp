@@ -393,7 +393,10 @@ __device__ __forceinline__ void ct_unit(const ZP* zb, const M& mod, uint32_t slo
 #pragma unroll
     for (int lam = 0; lam < m; lam++)
 #pragma unroll
-        for (int g = ONE0 ? 1 : 0; g < (1 << lam); g++) w[(1 << lam) - 1 + g] = zb[lam][g];
+        for (int g = ONE0 ? 1 : 0; g < (1 << lam); g++) {
+            // member-wise: zb may point into the constant address space
+            w[(1 << lam) - 1 + g] = TwP<V>{zb[lam][g].w, zb[lam][g].wp};
+        }
     V x[1 << m];
 #pragma unroll
     for (int v = 0; v < (1 << m); v++) x[v] = lds_ld<V>(slot0 ^ vsl[v]);
@@ -437,12 +440,13 @@ __device__ __forceinline__ void ct_round(ZP z, const M& mod, uint32_t l0, uint32
 #pragma unroll
     for (int v = 0; v < (1 << m); v++) vsl[v] = swz((uint32_t)v << (log_t + log_c)) * VB;
     const uint32_t lowmask = (1u << (log_t + log_c)) - 1u;
-    // (threadIdx.x, not ltid(): these rounds run in the one-tile kernels only,
-    // and the 62-bit instances spill with the opaque index)
+    // (threadIdx.x, not ltid(): the 62-bit instances spill with the opaque index)
     for (uint32_t u = threadIdx.x; u < units; u += kNT) {
         // row-major word of (base, c): u's low log_t + log_c bits stay, i0 moves up by m
         const uint32_t word = (u & lowmask) | ((u & ~lowmask) << m);
-        const uint32_t i0 = u >> (log_t + log_c);
+        // i0 < 2^l0: at l0 = 0 every unit reads the same table entries, and
+        // the constant lets them come in as scalar loads
+        const uint32_t i0 = l0 == 0 ? 0u : u >> (log_t + log_c);
         ZP zb[m];
 #pragma unroll
         for (int lam = 0; lam < m; lam++) zb[lam] = z + ((1u << (l0 + lam)) + (i0 << lam));
@@ -708,6 +712,9 @@ __device__ __forceinline__ void store_loop_step(const PassJob& J, const Mod32& m
                              ((int64_t)(col0 >> o_lbits) << J.sh_ohs) + ((int64_t)(col0 & omask) << J.sh_oc) +
                              (((cl >> o_lbits) << J.sh_ohs) + ((cl & omask) << J.sh_oc) + ((4u * m) << J.sh_or));
     const uint32_t sweeps = 1u << (log_r - 2u - log_tpc);
+    // (unrolled where the shape is fixed: straight-line stores let the
+    // compiler count them when it waits for older loads)
+#pragma unroll
     for (uint32_t u = 0; u < sweeps; u++) {
         const uint32_t su = l_slot ^ slot(u * kstep, 0);
         V x[4];
@@ -906,9 +913,16 @@ __device__ __forceinline__ void stage_tables(const PassJob& J, const Staged<type
     }
 }
 
+// Store kinds a pipelined instance fixes at compile time (pipe_store_kind): a
+// tile loop with a single store loop lets the compiler count the stores that
+// follow the next tile's loads, so the wait for those loads leaves the stores
+// in flight.  With several store loops under branches it takes the path with
+// the fewest stores and waits for all of them.
+enum PipeStore : int { PS_ANY = 0, PS_NONE = 1, PS_ROW = 2, PS_ROW_LZ = 3, PS_STEP3_LZ = 4 };
+
 // The Cooley-Tukey levels of the tile in LDS (VALU rounds, or with MF radix-16
 // rounds on the matrix cores), then the store with the pass's post-multiply.
-template <class M, class Sout, int MAXM, bool P4, int LR, int LC, bool MF>
+template <class M, class Sout, int MAXM, bool P4, int LR, int LC, bool MF, int PS = PS_ANY>
 __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<typename M::V>& t, uint32_t poly,
                                              uint32_t col0, bool full, const MfMat& mat) {
     using V = typename M::V;
@@ -937,8 +951,18 @@ __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<type
     const TwP<V>* tm = reinterpret_cast<const TwP<V>*>(J.tw_mid);
     const TwP<V>* th = reinterpret_cast<const TwP<V>*>(J.tw_hi);
     if (LR) {  // shape-specialised: vector stores, full tiles, LDS post tables, fixed direction
-        if (flags & JF_STORE_CFAST) store_dispatch<4, P4, true, LR, LC, M, Sout, 1>(J, mod, poly, col0, false, t.pl, t.pm, t.ph);
-        else store_dispatch<4, P4, true, LR, LC, M, Sout, 0>(J, mod, poly, col0, false, t.pl, t.pm, t.ph);
+        // (the branches of store_dispatch these kinds take; every kind but
+        // PS_ANY has one store direction: column-fast, row-fast for the step)
+        if constexpr (PS == PS_ROW || PS == PS_ROW_LZ) {
+            store_loop<4, 2, 2, 3, false, true, LR, LC, M, Sout, PS == PS_ROW_LZ, 1>(J, mod, poly, col0, t.pl, t.pm, t.ph);
+        } else if constexpr (PS == PS_STEP3_LZ && std::is_same<M, Mod32>::value) {
+            store_loop_step<3, true, LR, LC, Sout, true>(J, mod, poly, col0, t.pl, t.pm, t.ph);
+        } else if constexpr (PS == PS_NONE) {
+            store_loop<4, 0, 2, 0, false, false, LR, LC, M, Sout, false, 1>(J, mod, poly, col0, t.pl, t.pm, t.ph);
+        } else {
+            if (flags & JF_STORE_CFAST) store_dispatch<4, P4, true, LR, LC, M, Sout, 1>(J, mod, poly, col0, false, t.pl, t.pm, t.ph);
+            else store_dispatch<4, P4, true, LR, LC, M, Sout, 0>(J, mod, poly, col0, false, t.pl, t.pm, t.ph);
+        }
     } else if (P4 && (flags & JF_POST_LDS)) {
         if (flags & JF_VEC4_ST) store_dispatch<4, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, t.pl, t.pm, t.ph);
         else store_dispatch<1, P4, true, LR, LC, M, Sout>(J, mod, poly, col0, pred_s, t.pl, t.pm, t.ph);
@@ -1067,7 +1091,7 @@ __device__ __forceinline__ void tile_commit(const PassJob& J, const TileLoad<LR,
 // xcd_remap); the host launches a multiple of 8 workgroups on a tile count
 // divisible by 8.  At most 6 waves per SIMD: the prefetch registers must not
 // cost the third workgroup per CU.
-template <class M, class Sin, class Sout, int MAXM, bool P4, int LR, int LC, bool MF, bool LCF>
+template <class M, class Sin, class Sout, int MAXM, bool P4, int LR, int LC, bool MF, bool LCF, int PS>
 __device__ __forceinline__ void pipe_run(const PassJob& J) {
     using V = typename M::V;
     const uint32_t T = J.tiles_per_poly * (J.npoly ? J.npoly : 1u);
@@ -1096,18 +1120,34 @@ __device__ __forceinline__ void pipe_run(const PassJob& J) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     lds_barrier();
     const bool row_post = P4 && (J.flags & JF_POST_ROW);
+    // A row table of at most one entry per thread comes in with the next
+    // tile's loads (one 8-byte load, before the current tile's stores), so no
+    // load issued after those stores has to be waited for before the commit.
+    constexpr bool kRowPre = (1u << LR) <= kNT;
+    const TwP<V>* rows = reinterpret_cast<const TwP<V>*>(J.tw_row);
+    TwP<V> rowp{};
     for (;;) {
         const uint32_t ni = i + gs;
         const bool more = ni < per;
         uint32_t npl = 0, ncol0 = 0;
+        bool restage = false;
         if (more) {
             tile_of(ni, npl, ncol0);
             tile_issue<LR, LC, Sin, LCF>(J, npl, ncol0, L);
+            restage = row_post && (ncol0 >> J.row_shift) != (col0 >> J.row_shift);
+            if (kRowPre && restage && ltid() < (1u << LR))
+                rowp = gld(rows + ((size_t)(ncol0 >> J.row_shift) << LR) + ltid());
         }
-        ct_and_store<M, Sout, MAXM, P4, LR, LC, MF>(J, t, poly, col0, true, mat);
+        ct_and_store<M, Sout, MAXM, P4, LR, LC, MF, PS>(J, t, poly, col0, true, mat);
         if (!more) break;
         lds_barrier();  // the store has read the tile (and the row table)
-        if (row_post && (ncol0 >> J.row_shift) != (col0 >> J.row_shift)) stage_tables<M, P4, LR, MF>(J, t, ncol0, true);
+        if (restage) {
+            if (kRowPre) {
+                if (ltid() < (1u << LR)) t.pl.set(ltid(), rowp);
+            } else {
+                stage_tables<M, P4, LR, MF>(J, t, ncol0, true);
+            }
+        }
         tile_commit<LR, LC, Sin, M, LCF>(J, L);
         lds_barrier();
         i = ni;
@@ -1122,10 +1162,40 @@ __device__ __forceinline__ void pipe_run(const PassJob& J) {
 template <int LR, int LC>
 constexpr int pipe_waves() { return LR + LC >= 14 ? 4 : 6; }
 
+// The store kind of a pipelined job (PipeStore), as store_dispatch would
+// choose it for a full, shape-specialised tile; PS_ANY keeps the branches.
+template <bool P4>
+__device__ __forceinline__ int pipe_store_kind(const PassJob& J) {
+    const bool cf = J.flags & JF_STORE_CFAST;
+    if (!P4) return J.post == POST_NONE && cf ? PS_NONE : PS_ANY;
+    if (J.post != POST_FOURSTEP) return PS_ANY;
+    if (J.flags & JF_POST_ROW) return !cf ? PS_ANY : (J.flags & JF_LAZY_OUT) ? PS_ROW_LZ : PS_ROW;
+    if ((J.flags & JF_POST_STEP) && J.mid_bits && (J.flags & JF_LAZY_OUT) && !(J.flags & JF_STORE_CFAST))
+        return PS_STEP3_LZ;
+    return PS_ANY;
+}
+
 template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC, bool MF>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pipe_waves<LR, LC>()))) pipe_kernel(PassJob J) {
-    if (J.flags & JF_LOAD_CFAST) pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, true>(J);
-    else pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, false>(J);
+    if (!(J.flags & JF_LOAD_CFAST)) {
+        pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, false, PS_ANY>(J);
+        return;
+    }
+    switch (pipe_store_kind<P4>(J)) {
+        case PS_NONE:
+            if constexpr (!P4) pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, true, PS_NONE>(J);
+            break;
+        case PS_ROW:
+            if constexpr (P4) pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, true, PS_ROW>(J);
+            break;
+        case PS_ROW_LZ:
+            if constexpr (P4) pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, true, PS_ROW_LZ>(J);
+            break;
+        case PS_STEP3_LZ:
+            if constexpr (P4) pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, true, PS_STEP3_LZ>(J);
+            break;
+        default: pipe_run<M, Sin, Sout, MAXM, P4, LR, LC, MF, true, PS_ANY>(J); break;
+    }
 }
 
 // Workgroup index -> logical tile so that consecutive tiles run on the same

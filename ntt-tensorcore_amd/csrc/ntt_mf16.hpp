@@ -159,10 +159,15 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
 // global); cyc: cyclic table, so round 0's twiddles are all 1; canon_out:
 // the tile is stored without a post-multiply, so the outputs must be canonical.
 template <class ZP, class TP>
-__device__ __forceinline__ void mf_levels(const Mod32& mod, ZP zg, const MfMat& M, TP twl, uint32_t log_r,
+__device__ __forceinline__ void mf_levels(const Mod32& mod, ZP zg0, const MfMat& M, TP twl, uint32_t log_r,
                                           uint32_t log_c, bool cyc, bool canon_out) {
     const uint32_t m0 = log_r & 3u;
     if (m0) {
+        // The VALU round reads at most 7 entries, the same in every thread: read
+        // through the constant address space they are scalar loads, which no
+        // wait for the vector loads in flight (a pipelined tile's) holds up.
+        typedef const __attribute__((address_space(4))) TwP<uint32_t>* CZP;
+        const CZP zg = (CZP)zg0;
         switch (m0) {
             case 1: cyc ? ct_round<1, true, Mod32>(zg, mod, 0, log_r, log_c) : ct_round<1, false, Mod32>(zg, mod, 0, log_r, log_c); break;
             case 2: cyc ? ct_round<2, true, Mod32>(zg, mod, 0, log_r, log_c) : ct_round<2, false, Mod32>(zg, mod, 0, log_r, log_c); break;
