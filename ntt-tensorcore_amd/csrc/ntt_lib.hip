@@ -367,10 +367,12 @@ static int64_t centred(uint64_t v, uint64_t q) { return v > q / 2 ? (int64_t)v -
 // (layout in ntt_mf16.hpp).  From the merged CT table Z of the pass, the
 // round at level l0 maps unit i0's 16 slots by the 4-level butterfly network
 // of ct_unit<4>: x -> U_i0 x.  Every U_i0 factors as F diag(d_i0) with one F
-// for all rounds and units (F[p][v] = w16^(v*br4(p)), the network of block 0
-// without its factors); d_i0[v] = U_i0[0][v] since F[0][v] = 1.  Both are
-// read off the simulated network and the factorisation is checked for every
-// unit, so the MFMA rounds reproduce the VALU rounds' outputs exactly.
+// for all rounds and units (F[p][v] = w16^(v*br4(p)) times a per-column
+// factor); d_i0[v] = U_i0[0][v] / F[0][v].  Where the first round is a
+// matrix-core round (log_r mod 4 = 0), F is that round's own U, so its d is
+// all ones; otherwise F[0][v] = 1.  Both are read off the simulated network
+// and the factorisation is checked for every unit, so the MFMA rounds
+// reproduce the VALU rounds' outputs exactly.
 static bool build_mf16_block(uint32_t log_r, const std::vector<uint64_t>& Z, uint64_t q,
                              std::vector<unsigned char>& blk) {
     const uint32_t m0 = log_r & 3u;
@@ -396,20 +398,26 @@ static bool build_mf16_block(uint32_t log_r, const std::vector<uint64_t>& Z, uin
                 }
                 for (uint32_t p = 0; p < 16; p++) U[p][v] = x[p];
             }
-            uint64_t dinv[16];
-            for (uint32_t v = 0; v < 16; v++) {
-                if (U[0][v] == 0) return false;
-                tw.push_back(U[0][v]);
-                dinv[v] = invmod(U[0][v], q);
-            }
-            const bool first = F.empty();
-            if (first) F.resize(256);
-            for (uint32_t p = 0; p < 16; p++)
+            if (F.empty()) {
+                // With no VALU round first (m0 = 0), F is round 0's own
+                // matrix, so that round's twiddles are all 1 and the kernel
+                // skips them (a negacyclic first pass's weights ride in F);
+                // otherwise F is normalised to a first row of ones.
+                F.resize(256);
                 for (uint32_t v = 0; v < 16; v++) {
-                    const uint64_t f = mulmod(U[p][v], dinv[v], q);
-                    if (first) F[p * 16 + v] = f;
-                    else if (F[p * 16 + v] != f) return false;
+                    if (U[0][v] == 0) return false;
+                    const uint64_t dinv = invmod(U[0][v], q);
+                    for (uint32_t p = 0; p < 16; p++) F[p * 16 + v] = m0 == 0 ? U[p][v] : mulmod(U[p][v], dinv, q);
                 }
+            }
+            uint64_t d[16];
+            for (uint32_t v = 0; v < 16; v++) {
+                d[v] = mulmod(U[0][v], invmod(F[v], q), q);  // U = F diag(d): d = U's first row / F's
+                tw.push_back(d[v]);
+            }
+            for (uint32_t p = 0; p < 16; p++)
+                for (uint32_t v = 0; v < 16; v++)
+                    if (mulmod(F[p * 16 + v], d[v], q) != U[p][v]) return false;
         }
     }
     if (F.empty()) return false;

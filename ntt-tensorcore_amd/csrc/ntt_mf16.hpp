@@ -6,12 +6,13 @@
 // v*2^t, v < 16) to
 //     y = F * diag(d_i0) * x  (mod q),
 // where F[p][v] is the same 16x16 matrix in every round (the 4-level
-// butterfly network of a unit whose twiddles are those of block 0) and d_i0
-// is a per-unit twiddle vector (i0 = the unit's block index).  The host
+// butterfly network of block 0 of the first round, with its twiddles) and
+// d_i0 is a per-unit twiddle vector (i0 = the unit's block index).  The host
 // derives F and d from the very butterfly network ct_unit<4> runs, so the
 // output is the VALU round's, slot for slot.  A round here is therefore:
 //   1. each lane reads 4 words of one unit (lane (g, n): unit n, v = 4g..4g+3),
-//   2. multiplies them by d (lazy Shoup, any 32-bit representative),
+//   2. multiplies them by d (lazy Shoup, any 32-bit representative; skipped
+//      in a round at level 0, whose d is all ones),
 //   3. flips every byte to a signed digit (XOR 0x80: byte - 128) — the four
 //      words are the MFMA's B operand as they are: K slot (v, byte),
 //   4. four MFMAs, one per balanced signed byte plane F_c of
@@ -156,7 +157,7 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
 // All log_r levels of the tile: a VALU round of log_r mod 4 levels first
 // (merged CT table zg, as ct_levels), then radix-16 rounds on the matrix
 // cores.  M: the pass's matrix (mf_load_mat); twl: its round twiddle tables (LDS or
-// global); cyc: cyclic table, so round 0's twiddles are all 1; canon_out:
+// global); cyc: cyclic table, so the VALU round skips its products by 1; canon_out:
 // the tile is stored without a post-multiply, so the outputs must be canonical.
 template <class ZP, class TP>
 __device__ __forceinline__ void mf_levels(const Mod32& mod, ZP zg0, const MfMat& M, TP twl, uint32_t log_r,
@@ -182,7 +183,9 @@ __device__ __forceinline__ void mf_levels(const Mod32& mod, ZP zg0, const MfMat&
     uint32_t off = 0;
     for (uint32_t l0 = m0; l0 + 4u <= log_r; l0 += 4u) {
         const bool last = l0 + 8u > log_r;
-        const bool tw = l0 > 0 || !cyc;
+        // a first round at level 0 has all-one twiddles (the block's F is its
+        // own matrix, negacyclic weights included)
+        const bool tw = l0 > 0;
         const int cn = last ? canon : 0;
 #define NTT_MF_ROUND(T, C) mf_round<T, C>(M, twl + off, q, l0, log_r, log_c)
         if (tw) {
