@@ -922,9 +922,10 @@ enum PipeStore : int { PS_ANY = 0, PS_NONE = 1, PS_ROW = 2, PS_ROW_LZ = 3, PS_ST
 
 // The Cooley-Tukey levels of the tile in LDS (VALU rounds, or with MF radix-16
 // rounds on the matrix cores), then the store with the pass's post-multiply.
-template <class M, class Sout, int MAXM, bool P4, int LR, int LC, bool MF, int PS = PS_ANY>
+template <class M, class Sout, int MAXM, bool P4, int LR, int LC, bool MF, int PS = PS_ANY, bool PK = false>
 __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<typename M::V>& t, uint32_t poly,
-                                             uint32_t col0, bool full, const MfMat& mat) {
+                                             uint32_t col0, bool full, const MfMat& mat,
+                                             const MfMat* mat0 = nullptr) {
     using V = typename M::V;
     const uint32_t log_r = LR ? (uint32_t)LR : J.log_r, log_c = LR ? (uint32_t)LC : J.log_c;
     const uint32_t total = 1u << (log_r + log_c);
@@ -936,8 +937,8 @@ __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<type
     if constexpr (MF && std::is_same<M, Mod32>::value) {
         const TwP<V>* twg = reinterpret_cast<const TwP<V>*>(reinterpret_cast<const char*>(J.mz) + kMfTwOff);
         const bool canon = J.post == POST_NONE;
-        if (flags & JF_Z_LDS) mf_levels(mod, zg, mat, t.zl, log_r, log_c, cyc, canon);
-        else mf_levels(mod, zg, mat, twg, log_r, log_c, cyc, canon);
+        if (flags & JF_Z_LDS) mf_levels<PK>(mod, zg, mat, t.zl, log_r, log_c, cyc, canon, mat0);
+        else mf_levels<PK>(mod, zg, mat, twg, log_r, log_c, cyc, canon, mat0);
     } else if (LR) {
         if (flags & JF_Z_LDS) ct_levels_c<LR, LC, MAXM, 0, M>(t.zl, mod, cyc);
         else ct_levels_c<LR, LC, MAXM, 0, M>(zg, mod, cyc);
@@ -1106,12 +1107,18 @@ __device__ __forceinline__ void pipe_run(const PassJob& J) {
     };
     const Staged<V> t = staged_of<M, P4, LR, MF>(J);
     MfMat mat{};
+    // A remainder round of 2 or 3 levels runs packed on the matrix cores
+    // where the registers allow a second matrix (2^14-word tiles, 4 waves
+    // per SIMD); the one-tile kernels keep it on the VALU.
+    constexpr bool kPack = MF && std::is_same<M, Mod32>::value && (LR & 3) >= 2 && LR + LC >= 14;
+    MfMat mat0{};
     TileLoad<LR, LC, Sin> L;
     uint32_t poly, col0;
     tile_of(i, poly, col0);
     tile_issue<LR, LC, Sin, LCF>(J, poly, col0, L);
     stage_tables<M, P4, LR, MF>(J, t, col0, false);
     if constexpr (MF && std::is_same<M, Mod32>::value) mf_load_mat(J.mz, mat);
+    if constexpr (kPack) mf_load_mat(J.mz, mat0, mf_pack_off(LR));
     tile_commit<LR, LC, Sin, M, LCF>(J, L);
     // Every load before the loop (matrix fragments, tables) has landed.  The
     // loop's first use of the fragments would otherwise carry a wait of its
@@ -1138,7 +1145,7 @@ __device__ __forceinline__ void pipe_run(const PassJob& J) {
             if (kRowPre && restage && ltid() < (1u << LR))
                 rowp = gld(rows + ((size_t)(ncol0 >> J.row_shift) << LR) + ltid());
         }
-        ct_and_store<M, Sout, MAXM, P4, LR, LC, MF, PS>(J, t, poly, col0, true, mat);
+        ct_and_store<M, Sout, MAXM, P4, LR, LC, MF, PS, kPack>(J, t, poly, col0, true, mat, &mat0);
         if (!more) break;
         lds_barrier();  // the store has read the tile (and the row table)
         if (restage) {

@@ -421,18 +421,9 @@ static bool build_mf16_block(uint32_t log_r, const std::vector<uint64_t>& Z, uin
         }
     }
     if (F.empty()) return false;
-    blk.assign(kMfTwOff + tw.size() * 8, 0);
-    int8_t* frag = (int8_t*)blk.data();
-    int32_t* h = (int32_t*)(blk.data() + kMfFragBytes);
+    const bool pack = mf_has_pack(log_r);
+    blk.assign(pack ? mf_pack_off(log_r) + kMfPackBytes : kMfTwOff + tw.size() * 8, 0);
     uint32_t* hdr = (uint32_t*)(blk.data() + kMfHdrOff);
-    // G[p][(v, b)] = F[p][v] 2^(8b) mod q, K slot 4v + b; lane l holds K = 16 (l >> 4) + i, i < 16
-    auto G = [&](uint32_t p, uint32_t k) -> uint64_t { return mulmod(F[p * 16 + (k >> 2)], 1ull << (8 * (k & 3)), q); };
-    for (uint32_t lane = 0; lane < 64; lane++)
-        for (uint32_t i = 0; i < 16; i++) {
-            int8_t d[4];
-            balanced_digits(centred(G(lane & 15u, 16 * (lane >> 4) + i), q), d);
-            for (int c = 0; c < 4; c++) frag[(c * 64 + lane) * 16 + i] = d[c];
-        }
     // Row p: y = lo + r + L0 with r = hi * 2^16 (mod q) must represent
     // sum_K G[p][K] * byte_K; the digits are byte - 128, so plane 2 starts at
     // T'_p (centred) with L0 + 2^16 T'_p = 128 * rowsum_p (mod q).
@@ -444,13 +435,58 @@ static bool build_mf16_block(uint32_t log_r, const std::vector<uint64_t>& Z, uin
     hdr[0] = (uint32_t)L0;
     hdr[1] = (uint32_t)(int32_t)cm;
     hdr[2] = (uint32_t)(int32_t)cm * qinv;
-    for (uint32_t p = 0; p < 16; p++) {
-        uint64_t rs = 0;
-        for (uint32_t k = 0; k < 64; k++) rs = (rs + G(p, k)) % q;
-        const uint64_t T = mulmod(128, rs, q);
-        const int64_t t = centred(mulmod((T + q - L0 % q) % q, inv16, q), q);
+    // Fragments and plane-2 initial values of a 16 x 16 matrix A at byte
+    // offset off: G[p][(v, b)] = A[p][v] 2^(8b) mod q, K slot 4v + b; lane l
+    // holds K = 16 (l >> 4) + i, i < 16, and output rows 4 (l >> 4) .. + 3.
+    auto emit = [&](const std::vector<uint64_t>& A, uint32_t off) {
+        int8_t* frag = (int8_t*)(blk.data() + off);
+        int32_t* h = (int32_t*)(blk.data() + off + kMfFragBytes);
+        auto G = [&](uint32_t p, uint32_t k) -> uint64_t { return mulmod(A[p * 16 + (k >> 2)], 1ull << (8 * (k & 3)), q); };
         for (uint32_t lane = 0; lane < 64; lane++)
-            if (4 * (lane >> 4) <= p && p < 4 * (lane >> 4) + 4) h[lane * 4 + (p & 3)] = (int32_t)t;
+            for (uint32_t i = 0; i < 16; i++) {
+                int8_t d[4];
+                balanced_digits(centred(G(lane & 15u, 16 * (lane >> 4) + i), q), d);
+                for (int c = 0; c < 4; c++) frag[(c * 64 + lane) * 16 + i] = d[c];
+            }
+        for (uint32_t p = 0; p < 16; p++) {
+            uint64_t rs = 0;
+            for (uint32_t k = 0; k < 64; k++) rs = (rs + G(p, k)) % q;
+            const uint64_t T = mulmod(128, rs, q);
+            const int64_t t = centred(mulmod((T + q - L0 % q) % q, inv16, q), q);
+            for (uint32_t lane = 0; lane < 64; lane++)
+                if (4 * (lane >> 4) <= p && p < 4 * (lane >> 4) + 4) h[lane * 4 + (p & 3)] = (int32_t)t;
+        }
+    };
+    emit(F, 0);
+    if (pack) {
+        // The VALU remainder round (m0 levels at level 0, one block) as a
+        // block-diagonal matrix: 16 >> m0 units of 2^m0 slots per MFMA
+        // column, slot w = unit h * 2^m0 + v (the kernel's address map puts
+        // v in the high and h in the low bits of the 4-bit slot field).
+        const uint32_t S = 1u << m0;
+        std::vector<uint64_t> Um(S * S);
+        for (uint32_t v = 0; v < S; v++) {
+            uint64_t x[8] = {};
+            x[v] = 1;
+            for (uint32_t lam = 0; lam < m0; lam++) {
+                const uint32_t half = 1u << (m0 - 1 - lam);
+                for (uint32_t gg = 0; gg < (1u << lam); gg++) {
+                    const uint64_t wt = Z[(1u << lam) + gg];
+                    for (uint32_t v0 = 0; v0 < half; v0++) {
+                        const uint32_t j = gg * 2 * half + v0;
+                        const uint64_t a = x[j], bb = mulmod(x[j + half], wt, q);
+                        x[j] = (a + bb) % q;
+                        x[j + half] = (a + q - bb) % q;
+                    }
+                }
+            }
+            for (uint32_t p = 0; p < S; p++) Um[p * S + v] = x[p];
+        }
+        std::vector<uint64_t> A(256, 0);
+        for (uint32_t p = 0; p < 16; p++)
+            for (uint32_t w = 0; w < 16; w++)
+                if ((p >> m0) == (w >> m0)) A[p * 16 + w] = Um[(p & (S - 1)) * S + (w & (S - 1))];
+        emit(A, mf_pack_off(log_r));
     }
     uint32_t* twp = (uint32_t*)(blk.data() + kMfTwOff);
     for (size_t i = 0; i < tw.size(); i++) {

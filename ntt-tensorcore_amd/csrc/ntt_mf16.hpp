@@ -51,9 +51,11 @@ struct MfMat {
     uint32_t l0, cm, cmq;
 };
 
-__device__ __forceinline__ void mf_load_mat(const void* blk, MfMat& M) {
+// The matrix at byte offset off of the block (0: F; mf_pack_off: the packed
+// first round), with the block's shared header.
+__device__ __forceinline__ void mf_load_mat(const void* blk, MfMat& M, uint32_t off = 0) {
     const uint32_t lane = threadIdx.x & 63u;
-    const mf_v4i* f = reinterpret_cast<const mf_v4i*>(blk);
+    const mf_v4i* f = reinterpret_cast<const mf_v4i*>(reinterpret_cast<const char*>(blk) + off);
 #pragma unroll
     for (int c = 0; c < 4; c++) M.a[c] = gld(f + c * 64 + lane);
     M.h = gld(f + 256 + lane);
@@ -71,12 +73,27 @@ __host__ __device__ inline uint32_t mf_tw_entries(uint32_t log_r) {
     return n;
 }
 
+// A pass whose first m0 = 2 or 3 levels are a VALU remainder round can run
+// that round on the matrix cores too: 16 >> m0 units of 2^m0 slots share an
+// MFMA column under a block-diagonal matrix, stored after the round twiddle
+// tables (fragments and plane-2 initial values, the header shared).  Slot w =
+// unit h * 2^m0 + v of a column sits where a radix-16 unit at level 0 has its
+// slot rot(w) = v * 2^(4 - m0) + h: the v stride is the round's, and the
+// packed units are the neighbours just below it.
+__host__ __device__ inline bool mf_has_pack(uint32_t log_r) { return (log_r & 3u) >= 2u && log_r >= 4u + (log_r & 3u); }
+__host__ __device__ inline uint32_t mf_pack_off(uint32_t log_r) {
+    return (kMfTwOff + mf_tw_entries(log_r) * 8u + 255u) & ~255u;
+}
+constexpr uint32_t kMfPackBytes = kMfFragBytes + 64u * 16u;
+
 // One radix-16 round at level l0 over the whole tile (2^log_r rows, 2^log_c
 // columns, row-major and swizzled at LDS byte offset 0).  TW: multiply the
 // inputs by the round's twiddles tw[i0*16 + v].  CANON: 0 leaves any
 // representative below 2^32 (the next round, or a post-multiply, takes it),
 // 1 reduces by one conditional subtract (q >= 2^30), 2 by a Shoup reduction.
-template <bool TW, int CANON, class TP>
+// PK (2 or 3): the packed first round of that many levels (l0 = 0, no
+// twiddles, M the block-diagonal matrix); 0: a radix-16 round.
+template <bool TW, int CANON, class TP, int PK = 0>
 __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint32_t l0, uint32_t log_r,
                                          uint32_t log_c) {
     const uint32_t sh = log_r - l0 - 4u + log_c;  // log2 of the word stride of v within a unit
@@ -93,11 +110,15 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
     // scalar) ^ (group part U, a scalar): one vector XOR per access.
     // (The empty asm keeps the compiler from refactoring la ^ (se ^ U) back
     // into (word ^ word) * 4, more vector ops per access.)
-    uint32_t la = (swz(word(n)) ^ swz((4u * g) << sh)) * 4u;
+    // slot w -> position in the 4-bit slot field (identity for radix 16)
+    auto rot = [](uint32_t w) -> uint32_t {
+        return PK ? (((w & ((1u << PK) - 1u)) << (4 - PK)) | (w >> PK)) : w;
+    };
+    uint32_t la = (swz(word(n)) ^ swz(rot(4u * g) << sh)) * 4u;
     __asm__("" : "+v"(la));
     uint32_t se[4];
 #pragma unroll
-    for (int e = 0; e < 4; e++) se[e] = __builtin_amdgcn_readfirstlane(swz((uint32_t)e << sh) * 4u);
+    for (int e = 0; e < 4; e++) se[e] = __builtin_amdgcn_readfirstlane(swz(rot((uint32_t)e) << sh) * 4u);
     // twiddle index (((ub | n) >> sh) << 4) | 4g = lane part | (ub >> sh) << 4
     const uint32_t lti = ((n >> sh) << 4) | (4u * g);
     // with the table in LDS: byte address of the lane part
@@ -159,11 +180,17 @@ __device__ __forceinline__ void mf_round(const MfMat& M, TP tw, uint32_t q, uint
 // cores.  M: the pass's matrix (mf_load_mat); twl: its round twiddle tables (LDS or
 // global); cyc: cyclic table, so the VALU round skips its products by 1; canon_out:
 // the tile is stored without a post-multiply, so the outputs must be canonical.
-template <class ZP, class TP>
+// PK: the remainder round runs packed on the matrix cores with M0 (the
+// block's mf_pack_off matrix; log_r mod 4 = 2 or 3).
+template <bool PK = false, class ZP, class TP>
 __device__ __forceinline__ void mf_levels(const Mod32& mod, ZP zg0, const MfMat& M, TP twl, uint32_t log_r,
-                                          uint32_t log_c, bool cyc, bool canon_out) {
+                                          uint32_t log_c, bool cyc, bool canon_out, const MfMat* M0 = nullptr) {
     const uint32_t m0 = log_r & 3u;
-    if (m0) {
+    if (PK && m0 >= 2) {
+        if (m0 == 3) mf_round<false, 0, TP, 3>(*M0, twl, mod.q, 0, log_r, log_c);
+        else mf_round<false, 0, TP, 2>(*M0, twl, mod.q, 0, log_r, log_c);
+        lds_barrier();
+    } else if (m0) {
         // The VALU round reads at most 7 entries, the same in every thread: read
         // through the constant address space they are scalar loads, which no
         // wait for the vector loads in flight (a pipelined tile's) holds up.
