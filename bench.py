@@ -373,11 +373,14 @@ def extra_rns(ctx, stream):
             "cpu_baseline": _cpu_pool(bfly, cpu, "24 limbs x 2^16 negacyclic forward")}
 
 
-def extra_rns_kernels(stream):
+def extra_rns_kernels(ctx, stream):
     """RNS kernels around the transform on the 24 x 2^16 limb set: fast base
     conversion of 24 limbs to 24 other primes (ntt_c_bconv; 576 multiply-adds
-    and 192 HBM bytes per coefficient) and a 3-term per-limb multiply-accumulate
-    (ntt_c_pmac, dnum = 3, accumulating: 32 HBM bytes per output residue)."""
+    and 192 HBM bytes per coefficient), a 3-term per-limb multiply-accumulate
+    (ntt_c_pmac, dnum = 3, accumulating: 32 HBM bytes per output residue), and
+    that inner product fused into the 24 negacyclic inverse transforms it
+    feeds (ntt_c_batch_prepare_ip) against ntt_c_pmac followed by the inverse
+    batch."""
     n, L = 1 << 16, 24
     both = W.ntt_primes(1 << 31, 17, 2 * L)
     q, p = both[:L], both[L:]
@@ -392,15 +395,33 @@ def extra_rns_kernels(stream):
     b = nt.DeviceBuffer.from_array(np.concatenate([x[::-1]] * dnum))
     pm = nt.Pmac(q)
     t_p = timed(lambda: pm.run(dout, a, b, dnum, n, accumulate=True, stream=stream), 50, 5, stream) / 50 / 1e3
-    for o in (bc, pm):
+    # inner product -> inverse transform, fused and as two steps
+    flags = nt.FLAG_INVERSE | nt.FLAG_NEGACYCLIC
+    fused = nt.DeviceBuffer(L * n * 4)
+    bf = nt.Batch(ctx, [(fused.ptr + l * n * 4, q[l], 16, flags) for l in range(L)],
+                  ips=[(a.ptr + l * n * 4, b.ptr + l * n * 4, dnum, L * n) for l in range(L)])
+    bs = nt.Batch(ctx, [(dout.ptr + l * n * 4, q[l], 16, flags) for l in range(L)])
+    t_f = timed(lambda: bf.launch_graph(stream), 50, 5, stream) / 50 / 1e3
+
+    def two_steps():
+        pm.run(dout, a, b, dnum, n, accumulate=False, stream=stream)
+        bs.launch_graph(stream)
+
+    t_s = timed(two_steps, 50, 5, stream) / 50 / 1e3
+    stream.sync()
+    _require(np.array_equal(fused.download(np.uint32, L * n), dout.download(np.uint32, L * n)),
+             "rns_kernels: fused inner product + inverse differs from pmac then inverse")
+    for o in (bc, pm, bf, bs):
         o.destroy()
-    for buf in (din, dout, a, b):
+    for buf in (din, dout, a, b, fused):
         buf.free()
     bytes_b, bytes_p = n * 4 * (2 * L), n * L * 4 * (2 * dnum + 2)
     return {"bconv": {"L": L, "K": L, "n": n, "us": t_b * 1e6, "Gmac_s": n * L * L / t_b / 1e9,
                       "GB_s": bytes_b / t_b / 1e9, "frac": bytes_b / t_b / 1e9 / HBM_PEAK_GBS},
             "pmac": {"dnum": dnum, "L": L, "n": n, "us": t_p * 1e6, "GB_s": bytes_p / t_p / 1e9,
-                     "frac": bytes_p / t_p / 1e9 / HBM_PEAK_GBS}}
+                     "frac": bytes_p / t_p / 1e9 / HBM_PEAK_GBS},
+            "ip_inverse": {"dnum": dnum, "L": L, "n": n, "fused_us": t_f * 1e6, "pmac_then_inverse_us": t_s * 1e6,
+                           "speedup": t_s / t_f, "checked": "fused == pmac then inverse (bit-exact)"}}
 
 
 def extra_uneven(ctx, stream):
@@ -630,7 +651,7 @@ def main():
                  ("negacyclic_multiply", lambda: extra_polymul(stream)),
                  ("negacyclic_2_20", lambda: extra_2_20(ctx, stream)),
                  ("rns_24x2_16", lambda: extra_rns(ctx, stream)),
-                 ("rns_kernels", lambda: extra_rns_kernels(stream)),
+                 ("rns_kernels", lambda: extra_rns_kernels(ctx, stream)),
                  ("uneven_512", lambda: extra_uneven(ctx, stream)),
                  ("u64_abi", lambda: extra_u64_abi(args, stream)),
                  ("klimb", lambda: extra_klimb(stream)),

@@ -221,6 +221,28 @@ ntt_status ntt_c_transform_split(ntt_c_context* ctx, const ntt_c_desc* desc, uin
  * builds tables for every (q, n) in the batch, the device job lists and the
  * workspace.  The descriptors' data pointers are bound at prepare time. */
 ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count, ntt_c_batch** out);
+/* The key-switching inner product fused into inverse transforms: the
+ * co-optimization of IP kernels with the NTT that the reference names as
+ * absent (COOP, src/ntt_optimized.cu:510-511).  A batch as above, except that
+ * each descriptor i with ips[i].dnum > 0 (NTT_FLAG_INVERSE required) reads,
+ * in place of its data, the inner product
+ *     x[c] = sum_{k < dnum} a[c + k*kstride] * b[c + k*kstride] mod q
+ * of canonical residues in the descriptor's storage, and its inverse
+ * transform writes INTT(x) to descs[i].data: x is formed in the first pass's
+ * load and never stored.  dnum = 1 is a pointwise product (the second half of
+ * a negacyclic multiplication).  Needs dnum <= 64, kstride >= n when
+ * dnum > 1, reserved = 0, and data not overlapping a or b; ips[i].dnum = 0
+ * transforms descs[i].data as ntt_c_batch_prepare does.  The inputs are bound
+ * at prepare time like the data. */
+typedef struct ntt_c_ip {
+    const void* a;
+    const void* b;
+    uint32_t dnum;     /* pairs in the inner product; 0: none */
+    uint32_t reserved; /* 0 */
+    int64_t kstride;   /* elements from pair k to pair k + 1 */
+} ntt_c_ip;
+ntt_status ntt_c_batch_prepare_ip(ntt_c_context* ctx, const ntt_c_desc* descs, const ntt_c_ip* ips, size_t count,
+                                  ntt_c_batch** out);
 /* Launch a prepared batch: one kernel launch per (pass, arithmetic, storage
  * pair, round kind) group of its jobs (two launches for a 32-bit mix of one-
  * and two-pass plans), or, after ntt_c_batch_set_queue(batch, 1, 0), the

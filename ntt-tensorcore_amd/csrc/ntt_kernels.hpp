@@ -89,7 +89,8 @@ __device__ __forceinline__ uint64_t mont64(uint64_t x, uint64_t y, uint64_t q, u
 
 // a*b mod q of canonical residues for the product fused into a pass's load
 // (JF_MUL_IN), by arithmetic: a Montgomery product, then one by R^2 mod q to
-// remove R^-1 (Mod32, Mod64); the field's own product (ModGold).
+// remove R^-1 (Mod32, Mod64); the field's own product (ModGold).  An inner
+// product sums the canonical prod() terms with add() and applies fin() once.
 template <class M>
 struct LoadMul;
 struct Mod32;
@@ -100,9 +101,13 @@ struct LoadMul<Mod32> {
     uint32_t q, qinv, r2;
     __device__ __forceinline__ explicit LoadMul(uint64_t q_, uint64_t r2_)
         : q((uint32_t)q_), qinv(inv_mod_2_32((uint32_t)q_)), r2((uint32_t)r2_) {}
-    __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const {
-        return mont32(mont32(a, b, q, qinv), r2, q, qinv);
+    __device__ __forceinline__ uint32_t prod(uint32_t a, uint32_t b) const { return mont32(a, b, q, qinv); }
+    __device__ __forceinline__ uint32_t add(uint32_t a, uint32_t b) const {  // q < 2^31: no wrap
+        const uint32_t s = a + b;
+        return min(s, s - q);
     }
+    __device__ __forceinline__ uint32_t fin(uint32_t x) const { return mont32(x, r2, q, qinv); }
+    __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return fin(prod(a, b)); }
 };
 template <>
 struct LoadMul<Mod64> {
@@ -113,9 +118,13 @@ struct LoadMul<Mod64> {
         for (int i = 0; i < 5; i++) x *= 2u - q_ * x;
         qneg_inv = 0 - x;
     }
-    __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const {
-        return mont64(mont64(a, b, q, qneg_inv), r2, q, qneg_inv);
+    __device__ __forceinline__ uint64_t prod(uint64_t a, uint64_t b) const { return mont64(a, b, q, qneg_inv); }
+    __device__ __forceinline__ uint64_t add(uint64_t a, uint64_t b) const {  // q < 2^63: no wrap
+        const uint64_t s = a + b;
+        return s >= q ? s - q : s;
     }
+    __device__ __forceinline__ uint64_t fin(uint64_t x) const { return mont64(x, r2, q, qneg_inv); }
+    __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return fin(prod(a, b)); }
 };
 
 // q < 2^62: 2q < 2^63, Shoup with a 64-bit quotient.
@@ -193,10 +202,17 @@ struct ModGold {
 template <>
 struct LoadMul<ModGold> {
     __device__ __forceinline__ explicit LoadMul(uint64_t, uint64_t) {}
-    __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const {
+    __device__ __forceinline__ uint64_t prod(uint64_t a, uint64_t b) const {
         const uint64_t r = gold_mul_any(a, b);
         return r >= kGoldilocks ? r - kGoldilocks : r;
     }
+    // a + b < 2q: a carry out of 64 bits leaves a + b - 2^64 + (2^32 - 1) = a + b - q < q
+    __device__ __forceinline__ uint64_t add(uint64_t a, uint64_t b) const {
+        const uint64_t s = a + b;
+        return s < a ? s + 0xFFFFFFFFull : (s >= kGoldilocks ? s - kGoldilocks : s);
+    }
+    __device__ __forceinline__ uint64_t fin(uint64_t x) const { return x; }
+    __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return prod(a, b); }
 };
 
 enum PostMode : uint32_t { POST_NONE = 0, POST_FOURSTEP = 2, POST_SCALE = 3 };
@@ -241,7 +257,7 @@ struct PassJob {
     const void* tw_row;   // row table: factor of (u, k) at [u*R + k], u = first column >> row_shift
     const void* tw_col;   // column table of JF_POST_STEP: 4 Shoup pairs per column
     const void* mz;       // JF_MFMA: matrix block of the radix-16 rounds (ntt_mf16.hpp)
-    const void* in2;      // JF_MUL_IN: second factor, same layout and storage as in
+    const void* in2;      // JF_MUL_IN: second factor, same layout and storage as in (first pair)
     uint64_t q;
     uint64_t mul_r2;      // JF_MUL_IN: R^2 mod q, R = 2^(8*sizeof(V)) (undoes the Montgomery factor)
     uint64_t scale_w, scale_wp;  // POST_SCALE constant; with JF_POST_STEP, 2^32 mod q (Shoup pair)
@@ -266,7 +282,11 @@ struct PassJob {
     // have completed; dep_tiles = 0: no dependency.  counted: the job's tiles
     // count their completions (a later job depends on it, or it has a
     // dependency itself).
-    uint32_t dep, dep_tiles, counted, pad2_;
+    uint32_t dep, dep_tiles, counted;
+    // JF_MUL_IN: the load sums ip_n >= 1 products in[i + k*ip_stride] *
+    // in2[i + k*ip_stride], k < ip_n (an inner product; 0 counts as 1)
+    uint32_t ip_n;
+    int64_t ip_stride;
 };
 
 // Threads of every pass-kernel workgroup (a compile-time constant so that the
@@ -779,7 +799,8 @@ __device__ __forceinline__ void store_dispatch(const PassJob& J, const M& mod, u
 
 // Tile load into LDS (row-major, swizzled): PER elements per thread and
 // sweep, PRED per-element bounds predicates, LR/LC as in store_loop; MUL
-// stores in[i]*in2[i] mod q (JF_MUL_IN).
+// stores in[i]*in2[i] mod q, or the inner product of J.ip_n such pairs
+// (JF_MUL_IN).
 //   pre() runs once, after the first batch's global loads are issued and before
 //   their LDS stores, so that loads it issues share their latency.
 template <int PER, bool PRED, int LR, int LC, class Sin, class M, bool MUL = false, int CF = -1, class Pre>
@@ -853,6 +874,55 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
                 V x = (V)v[u][e];
                 if (MUL) x = mul(x, (V)w[MUL ? u : 0][e]);
                 lds_st<V>(s ^ e_x[e], x);
+            }
+        }
+    }
+    // An inner product's further pairs (k >= 1) add to the slots this thread
+    // has just written (no barrier: every slot has one writer): two sweeps and
+    // up to KP pairs per step, all their loads issued before the first is
+    // used, few enough registers that the loop above stays the kernel's peak.
+    // Only the generic instances (LR = 0) run inner products; the host gives
+    // shape-specialised launches single products only.
+    if (MUL && LR == 0 && J.ip_n > 1) {
+        constexpr uint32_t KP = sizeof(Sin) == 4 ? 2u : 1u;
+        for (uint32_t k = 1; k < J.ip_n; k += KP) {
+            const uint32_t kn = J.ip_n - k < KP ? J.ip_n - k : KP;
+#pragma unroll 1
+            for (uint32_t bb = 0; bb < total; bb += 2u * sweep) {
+                Sin av[KP][2][PER], bv[KP][2][PER];
+#pragma unroll
+                for (uint32_t p = 0; p < KP; p++) {
+                    if (p >= kn) break;
+                    const int64_t ko = (int64_t)(k + p) * J.ip_stride;
+#pragma unroll
+                    for (int u = 0; u < 2; u++) {
+                        const uint32_t ui = bb + (uint32_t)u * sweep;
+                        if (ui >= total) break;
+                        uint32_t ur, uc;
+                        rc(ui, ur, uc);
+                        const uint32_t off = goff(ur, uc) + l_off;
+                        const bool ok = !PRED || (lane_ok && col0 + (uc | lc) < ncols);
+                        fetch(in + ko + off, ok, av[p][u]);
+                        fetch(in2 + ko + off, ok, bv[p][u]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const uint32_t ui = bb + (uint32_t)u * sweep;
+                    if (ui >= total) break;
+                    if (PRED && !lane_ok) continue;
+                    uint32_t ur, uc;
+                    rc(ui, ur, uc);
+                    const uint32_t s = l_slot ^ slot(ur, uc);
+#pragma unroll
+                    for (int e = 0; e < PER; e++) {
+                        V x = lds_ld<V>(s ^ e_x[e]);
+#pragma unroll
+                        for (uint32_t p = 0; p < KP; p++)
+                            if (p < kn) x = mul.add(x, mul((V)av[p][u][e], (V)bv[p][u][e]));
+                        lds_st<V>(s ^ e_x[e], x);
+                    }
+                }
             }
         }
     }

@@ -646,7 +646,8 @@ static void finish_flags(PassJob& J, Arith ar, Store sin, Store sout, bool mfma)
     const bool scf = J.o_c == 1 && J.o_lbits >= 2;
     if (scf) f |= JF_STORE_CFAST;
     const int64_t ib = (int64_t)sbytes(sin), ob = (int64_t)sbytes(sout);
-    bool vl = aligned16(J.in) && (!J.in2 || aligned16(J.in2)) && (J.in_pstride * ib) % 16 == 0;
+    bool vl = aligned16(J.in) && (!J.in2 || aligned16(J.in2)) && (J.in_pstride * ib) % 16 == 0 &&
+              (J.ip_n <= 1 || (J.ip_stride * ib) % 16 == 0);
     if (lcf) vl = vl && C >= 4 && (J.s_r % 4) == 0 && (J.ncols % 4) == 0;
     else vl = vl && J.s_r == 1 && R >= 4 && (J.s_c % 4) == 0;
     if (vl) f |= JF_VEC4;
@@ -716,10 +717,11 @@ static void set_shifts(PassJob& J) {
 // elements, or dstride with two or more passes), with workspace at `work`
 // (stride n, arithmetic width).  With mul_a, the first pass reads
 // mul_a[i]*mul_b[i] mod q (canonical residues, same layout and storage as
-// data) instead of data.
+// data) instead of data, or with ip_n > 1 the inner product of the ip_n pairs
+// mul_a[i + k*ip_stride]*mul_b[i + k*ip_stride].
 static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void* work, PassJob out[4],
                       Store sin_out[4][2], const void* mul_a = nullptr, const void* mul_b = nullptr,
-                      int64_t dstride = 0) {
+                      int64_t dstride = 0, uint32_t ip_n = 1, int64_t ip_stride = 0) {
     const uint32_t log_n = P.log_n;
     const int64_t N = (int64_t)1 << log_n;
     if (dstride == 0) dstride = N;
@@ -759,6 +761,8 @@ static void make_jobs(const Plan& P, void* data, Store st, uint64_t npoly, void*
         if (i == 0 && mul_a) {
             J.in = mul_a;
             J.in2 = mul_b;
+            J.ip_n = ip_n;
+            J.ip_stride = ip_stride;
             const u128 r = P.arith == Arith::M32 ? ((u128)1 << 32) % P.q : ((u128)1 << 64) % P.q;
             J.mul_r2 = (uint64_t)(r * r % P.q);
         }
@@ -905,8 +909,9 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, co
 // or last pass on uint64 data; the first may carry the fused product).
 static uint32_t pass_shape_of(const PassJob& J, Arith ar, Store sin, Store sout) {
     if (ar != Arith::M32) return 0;
-    // the fused product's load: uint64 first passes only (see below)
-    if ((J.flags & JF_MUL_IN) && sin != Store::U64) return 0;
+    // the fused product's load: uint64 first passes only (see below), and
+    // single products only (inner products run in the generic instances)
+    if ((J.flags & JF_MUL_IN) && (sin != Store::U64 || J.ip_n > 1)) return 0;
     if (!(J.flags & JF_VEC4) || !(J.flags & JF_VEC4_ST)) return 0;
     if (J.post == POST_FOURSTEP && !(J.flags & JF_POST_LDS)) return 0;
     if (J.ncols % (1u << J.log_c) != 0) return 0;
@@ -1797,8 +1802,30 @@ static std::vector<PassJob> merge_jobs(const std::vector<PassJob>& jobs, Store s
     return out;
 }
 
-extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count,
-                                          ntt_c_batch** out) {
+// An inner-product source of ntt_c_batch_prepare_ip, checked before any
+// device work: an inverse transform, dnum <= 64 pairs of non-null slices
+// kstride >= n elements apart, none overlapping the descriptor's data.
+static ntt_status check_ip(const ntt_c_desc& d, const ntt_c_ip& ip) {
+    if (ip.dnum == 0) return NTT_OK;
+    if (ip.dnum > 64 || ip.reserved) return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: need dnum <= 64, reserved = 0");
+    if (!(d.flags & NTT_FLAG_INVERSE))
+        return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: an inner product feeds an inverse transform only");
+    if (!ip.a || !ip.b) return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: null inner-product input");
+    const int64_t n = (int64_t)1 << d.log_n;
+    if (ip.dnum > 1 && ip.kstride < n)
+        return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: slices closer than n elements (kstride < n)");
+    const uint64_t sb = (d.flags & NTT_FLAG_U64) ? 8 : 4;
+    const uint64_t span = ((uint64_t)(ip.dnum - 1) * (uint64_t)(ip.dnum > 1 ? ip.kstride : 0) + (uint64_t)n) * sb;
+    const uint64_t o = (uint64_t)(uintptr_t)d.data, ob = (uint64_t)n * sb;
+    for (const void* src : {ip.a, ip.b}) {
+        const uint64_t s = (uint64_t)(uintptr_t)src;
+        if (s < o + ob && o < s + span) return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: data overlaps an input");
+    }
+    return NTT_OK;
+}
+
+static ntt_status batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, const ntt_c_ip* ips, size_t count,
+                                ntt_c_batch** out) {
     if (!ctx || !out || (count && !descs)) return fail(NTT_ERR_INVALID_ARG, "batch_prepare: null argument");
     *out = nullptr;
     auto B = std::make_unique<ntt_c_batch>();
@@ -1809,6 +1836,7 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
     uint64_t work_bytes = 0, elems = 0;
     for (size_t i = 0; i < count; i++) {
         ntt_status st = check_desc(descs[i]);
+        if (st == NTT_OK && ips) st = check_ip(descs[i], ips[i]);
         if (st != NTT_OK) return st;
         elems += 1ull << descs[i].log_n;
     }
@@ -1859,7 +1887,9 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
         }
         PassJob jobs[4];
         Store io[4][2];
-        make_jobs(P, descs[i].data, st, 1, scr, jobs, io);
+        const ntt_c_ip* ip = ips && ips[i].dnum ? &ips[i] : nullptr;
+        make_jobs(P, descs[i].data, st, 1, scr, jobs, io, ip ? ip->a : nullptr, ip ? ip->b : nullptr, 0,
+                  ip ? ip->dnum : 1u, ip ? ip->kstride : 0);
         for (int k = 0; k < P.npass; k++) {
             items.push_back({jobs[k], P.arith, io[k][0], io[k][1], k});
             qpolys[i].push_back({jobs[k], P.arith, io[k][0], io[k][1]});
@@ -1921,6 +1951,17 @@ extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* 
     }
     *out = B.release();
     return NTT_OK;
+}
+
+extern "C" ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size_t count,
+                                          ntt_c_batch** out) {
+    return batch_prepare(ctx, descs, nullptr, count, out);
+}
+
+extern "C" ntt_status ntt_c_batch_prepare_ip(ntt_c_context* ctx, const ntt_c_desc* descs, const ntt_c_ip* ips,
+                                             size_t count, ntt_c_batch** out) {
+    if (count && !ips) return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: null inner-product array");
+    return batch_prepare(ctx, descs, ips, count, out);
 }
 
 static ntt_status batch_enqueue(ntt_c_batch* B, hipStream_t stream) {

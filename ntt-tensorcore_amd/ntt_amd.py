@@ -46,6 +46,12 @@ class Desc(ctypes.Structure):
                 ("flags", ctypes.c_uint32)]
 
 
+class IpSrc(ctypes.Structure):
+    """ntt_c_ip: the inner product an inverse transform of a batch reads."""
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("dnum", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("kstride", ctypes.c_int64)]
+
+
 _lib = None
 
 
@@ -95,6 +101,7 @@ def lib():
         "ntt_c_transform_split": [vp, P(Desc), u32, P(u32), u32, u32, vp],
         "ntt_c_transform_each": [vp, P(Desc), ctypes.c_size_t, vp],
         "ntt_c_batch_prepare": [vp, P(Desc), ctypes.c_size_t, P(vp)],
+        "ntt_c_batch_prepare_ip": [vp, P(Desc), P(IpSrc), ctypes.c_size_t, P(vp)],
         "ntt_c_batch_launch": [vp, vp],
         "ntt_c_batch_launch_graph": [vp, vp],
         "ntt_c_batch_set_queue": [vp, i32, u32],
@@ -476,11 +483,22 @@ class Context:
 
 
 class Batch:
-    def __init__(self, ctx, descs):
-        """descs: list of (data_ptr, q, log_n, flags)."""
+    def __init__(self, ctx, descs, ips=None):
+        """descs: list of (data_ptr, q, log_n, flags).  ips (optional, one per
+        descriptor): None, or (a_ptr, b_ptr, dnum, kstride) -- the inverse
+        transform of that descriptor reads the inner product
+        sum_k a[c + k*kstride] * b[c + k*kstride] mod q instead of its data
+        (ntt_c_batch_prepare_ip)."""
         arr = (Desc * len(descs))(*[Desc(*d) for d in descs])
         p = ctypes.c_void_p()
-        check(lib().ntt_c_batch_prepare(ctx.ptr, arr, len(descs), ctypes.byref(p)))
+        if ips is None:
+            check(lib().ntt_c_batch_prepare(ctx.ptr, arr, len(descs), ctypes.byref(p)))
+        else:
+            if len(ips) != len(descs):
+                raise ValueError("ips: one entry per descriptor")
+            src = (IpSrc * len(descs))(*[IpSrc(None, None, 0, 0, 0) if ip is None else
+                                         IpSrc(ip[0], ip[1], ip[2], 0, ip[3]) for ip in ips])
+            check(lib().ntt_c_batch_prepare_ip(ctx.ptr, arr, src, len(descs), ctypes.byref(p)))
         self.ptr = p.value
 
     def launch(self, stream=None):

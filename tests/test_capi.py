@@ -82,6 +82,32 @@ def test_descriptor_validation_is_host_side(nt):
     ctx.destroy()
 
 
+def test_inner_product_batch_checked_on_the_host(nt):
+    """ntt_c_batch_prepare_ip refuses a bad inner-product source before any
+    device work: a forward transform, a null input, more than 64 pairs,
+    slices closer than n elements, data overlapping an input."""
+    ctx = nt.Context(workspace_bytes=0)
+    n, inv = 1 << 10, nt.FLAG_INVERSE
+    a, b, out = 0x200000, 0x400000, 0x100000
+    cases = [
+        ((out, BB, 10, 0), (a, b, 2, n)),                  # forward transform
+        ((out, BB, 10, inv), (0, b, 2, n)),                # null a
+        ((out, BB, 10, inv), (a, 0, 1, 0)),                # null b
+        ((out, BB, 10, inv), (a, b, 65, n)),               # more than 64 pairs
+        ((out, BB, 10, inv), (a, b, 2, n - 1)),            # slices overlap
+        ((a + 4 * n, BB, 10, inv), (a, b, 2, n)),          # data is a's second slice
+        ((b - 4, BB, 10, inv), (a, b, 1, 0)),              # data runs into b
+        ((a + 8 * n, BB, 10, inv | nt.FLAG_U64), (a, b, 2, n)),  # 8-byte slices: data inside a
+    ]
+    for d, ip in cases:
+        with pytest.raises(nt.NttError) as e:
+            nt.Batch(ctx, [d], ips=[ip])
+        assert e.value.status == 1, (d, ip)
+    with pytest.raises(ValueError):
+        nt.Batch(ctx, [(out, BB, 10, inv)], ips=[])
+    ctx.destroy()
+
+
 def test_null_arguments_rejected(nt):
     L = nt.lib()
     assert L.ntt_c_forward(None, None, 3, None) == 1

@@ -112,3 +112,88 @@ def test_rns_plans_reject_bad_moduli(nt, orc):
         nt.Bconv(_primes(orc, 33), q)               # more than 32 source moduli
     with pytest.raises(nt.NttError):
         nt.Pmac([(1 << 31) + 11])                   # above 2^31
+
+
+def _ip_expect(orc, a, b, q, nega):
+    """Per limb l: the inverse transform of sum_k a[k][l] * b[k][l] mod q_l
+    (the oracle's pmac, then its inverse transform)."""
+    x = orc.pmac(a.astype(U32), b.astype(U32), q, None)
+    return np.stack([np.asarray(orc.transform(x[l].astype(np.uint64), q[l], nega, True), dtype=np.uint64)
+                     for l in range(len(q))])
+
+
+# The key-switching inner product fused into the inverse transforms of a batch
+# (ntt_c_batch_prepare_ip): limb l of the output is INTT_{q_l}(sum_k a[k][l] b[k][l]),
+# with a and b laid out [k][l][c] as for ntt_c_pmac.  One-pass (2^10, 2^12) and
+# multi-pass plans (2^16, 2^18), 32- and 64-bit storage, dnum = 1 (a pointwise
+# product) to 5.
+@pytest.mark.parametrize("u64", [False, True])
+@pytest.mark.parametrize("nega", [True, False])
+@pytest.mark.parametrize("dnum,L,log_n", [(1, 2, 10), (3, 4, 12), (3, 6, 16), (5, 2, 18), (2, 3, 4)])
+def test_ip_fused_inverse_vs_oracle(nt, orc, dnum, L, log_n, nega, u64):
+    n = 1 << log_n
+    q = [int(v) for v in orc.ntt_primes(1 << 31, max(17, log_n + 1), L)]
+    rng = np.random.default_rng(dnum * 1000 + L * 10 + log_n)
+    a = np.stack([_residues(rng, q, n) for _ in range(dnum)])
+    b = np.stack([_residues(rng, q, n) for _ in range(dnum)])
+    dt, sb = (np.uint64, 8) if u64 else (U32, 4)
+    flags = nt.FLAG_INVERSE | (nt.FLAG_NEGACYCLIC if nega else 0) | (nt.FLAG_U64 if u64 else 0)
+    da, db = nt.DeviceBuffer.from_array(a.astype(dt)), nt.DeviceBuffer.from_array(b.astype(dt))
+    dout = nt.DeviceBuffer(L * n * sb)
+    ctx = nt.Context()
+    bt = nt.Batch(ctx, [(dout.ptr + l * n * sb, q[l], log_n, flags) for l in range(L)],
+                  ips=[(da.ptr + l * n * sb, db.ptr + l * n * sb, dnum, L * n) for l in range(L)])
+    expect = _ip_expect(orc, a, b, q, nega)
+    for rep in range(2):  # the inputs are read, never overwritten: a second launch gives the same
+        bt.launch()
+        nt.device_sync()
+        got = dout.download(dt, L * n).reshape(L, n).astype(np.uint64)
+        for l in range(L):
+            assert np.array_equal(got[l], expect[l]), (rep, l)
+    bt.destroy()
+    ctx.destroy()
+    for x in (da, db, dout):
+        x.free()
+
+
+def test_ip_fused_inverse_equals_pmac_then_inverse(nt, orc):
+    """The RNS shape (24 limbs x 2^16, dnum = 3, negacyclic): the fused batch,
+    replayed from a graph and as one work-queue launch, equals ntt_c_pmac
+    followed by a batch of inverse transforms, and both equal the oracle on
+    sampled limbs."""
+    L, dnum, log_n = 24, 3, 16
+    n = 1 << log_n
+    q = [int(v) for v in orc.ntt_primes(1 << 31, 17, L)]
+    rng = np.random.default_rng(7)
+    a = np.stack([_residues(rng, q, n) for _ in range(dnum)])
+    b = np.stack([_residues(rng, q, n) for _ in range(dnum)])
+    flags = nt.FLAG_INVERSE | nt.FLAG_NEGACYCLIC
+    da, db = nt.DeviceBuffer.from_array(a), nt.DeviceBuffer.from_array(b)
+    fused, sep = nt.DeviceBuffer(L * n * 4), nt.DeviceBuffer(L * n * 4)
+    ctx = nt.Context()
+    bf = nt.Batch(ctx, [(fused.ptr + l * n * 4, q[l], log_n, flags) for l in range(L)],
+                  ips=[(da.ptr + l * n * 4, db.ptr + l * n * 4, dnum, L * n) for l in range(L)])
+    bs = nt.Batch(ctx, [(sep.ptr + l * n * 4, q[l], log_n, flags) for l in range(L)])
+    pm = nt.Pmac(q)
+    for _ in range(3):
+        bf.launch_graph()
+    pm.run(sep, da, db, dnum, n, accumulate=False)
+    bs.launch()
+    nt.device_sync()
+    got_f = fused.download(U32, L * n).reshape(L, n)
+    got_s = sep.download(U32, L * n).reshape(L, n)
+    assert np.array_equal(got_f, got_s)
+    expect = _ip_expect(orc, a[:, [0, 11, 23]], b[:, [0, 11, 23]], [q[0], q[11], q[23]], True)
+    for i, l in enumerate([0, 11, 23]):
+        assert np.array_equal(got_f[l].astype(np.uint64), expect[i]), l
+    # the same batch as one work-queue launch
+    fused.upload(np.zeros(L * n, dtype=U32), 0)
+    bf.set_queue(True)
+    bf.launch()
+    bf.check()
+    assert np.array_equal(fused.download(U32, L * n).reshape(L, n), got_s)
+    for x in (bf, bs, pm):
+        x.destroy()
+    ctx.destroy()
+    for x in (da, db, fused, sep):
+        x.free()
