@@ -881,9 +881,10 @@ __device__ __forceinline__ void load_loop(const PassJob& J, uint32_t poly, uint3
     // has just written (no barrier: every slot has one writer): two sweeps and
     // up to KP pairs per step, all their loads issued before the first is
     // used, few enough registers that the loop above stays the kernel's peak.
-    // Only the generic instances (LR = 0) run inner products; the host gives
-    // shape-specialised launches single products only.
-    if (MUL && LR == 0 && J.ip_n > 1) {
+    // Inner products run in the generic instances (LR = 0) and in the IP
+    // instances of run_pass; the other shape-specialised loads (uint64 first
+    // passes) take single products only.
+    if (MUL && (LR == 0 || sizeof(Sin) == 4) && J.ip_n > 1) {
         constexpr uint32_t KP = sizeof(Sin) == 4 ? 2u : 1u;
         for (uint32_t k = 1; k < J.ip_n; k += KP) {
             const uint32_t kn = J.ip_n - k < KP ? J.ip_n - k : KP;
@@ -1043,7 +1044,10 @@ __device__ __forceinline__ void ct_and_store(const PassJob& J, const Staged<type
     }
 }
 
-template <class M, class Sin, class Sout, int MAXM, bool P4, int LR, int LC, bool MF>
+// IP: an instance for jobs whose load forms a product or inner product
+// (JF_MUL_IN) on 32-bit shape-specialised tiles; kept apart from the plain
+// instances, whose registers the inner-product load would raise.
+template <class M, class Sin, class Sout, int MAXM, bool P4, int LR, int LC, bool MF, bool IP = false>
 __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job) {
     using V = typename M::V;
     const uint32_t poly = tile_in_job / J.tiles_per_poly;
@@ -1073,6 +1077,8 @@ __device__ __forceinline__ void run_pass(const PassJob& J, uint32_t tile_in_job)
         if constexpr (std::is_same<Sin, uint64_t>::value) {
             if (flags & JF_MUL_IN) load_loop<4, false, LR, LC, Sin, M, true>(J, poly, col0, pre);
             else load_loop<4, false, LR, LC, Sin, M>(J, poly, col0, pre);
+        } else if constexpr (IP) {
+            load_loop<4, false, LR, LC, Sin, M, true>(J, poly, col0, pre);
         } else {
             if (flags & JF_LOAD_CFAST) load_loop<4, false, LR, LC, Sin, M, false, 1>(J, poly, col0, pre);
             else load_loop<4, false, LR, LC, Sin, M, false, 0>(J, poly, col0, pre);
@@ -1304,17 +1310,17 @@ __device__ __forceinline__ uint32_t find_job(const PassJob* __restrict__ jobs, u
 // instance holds only one kind of round, so its registers are those of that
 // kind.  (Forcing 8 waves per SIMD on the matrix-core shapes, 4 workgroups
 // per CU, fits without scratch but measured ~2% slower at 2^24.)
-template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC, bool MF>
+template <class M, class Sin, class Sout, int MAXM, int NT, bool P4, int LR, int LC, bool MF, bool IP = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(M::kMinWaves))) pass_kernel(const PassJob* __restrict__ jobs, uint32_t njobs, PassJob inl) {
     // The dynamic LDS (no static LDS: it starts at address 0) is addressed by
     // byte offset through lds_ld / lds_st.
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     if (njobs == 0) {
-        run_pass<M, Sin, Sout, MAXM, P4, LR, LC, MF>(inl, tile);
+        run_pass<M, Sin, Sout, MAXM, P4, LR, LC, MF, IP>(inl, tile);
         return;
     }
     const PassJob J = jobs[find_job(jobs, njobs, tile)];
-    run_pass<M, Sin, Sout, MAXM, P4, LR, LC, MF>(J, tile - J.tile_begin);
+    run_pass<M, Sin, Sout, MAXM, P4, LR, LC, MF, IP>(J, tile - J.tile_begin);
 }
 
 // ---- a whole multi-pass batch in one persistent launch ----

@@ -15,10 +15,10 @@ constexpr uint32_t kPassLdsMax = 81920;  // LDS bytes a pass workgroup may use
 
 // Launches pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC, MF> with one
 // workgroup per tile (LR = 0: the generic kernel; MF: matrix-core rounds).
-template <class M, class Sin, class Sout, bool P4, int LR = 0, int LC = 0, bool MF = false>
+template <class M, class Sin, class Sout, bool P4, int LR = 0, int LC = 0, bool MF = false, bool IP = false>
 hipError_t launch_pass_kernel(const PassJob* djobs, uint32_t njobs, const PassJob& inl, uint32_t ntiles, uint32_t lds,
                               hipStream_t stream) {
-    auto fn = pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC, MF>;
+    auto fn = pass_kernel<M, Sin, Sout, 4, kPassThreads, P4, LR, LC, MF, IP>;
     static std::once_flag once;
     static hipError_t ready = hipSuccess;
     std::call_once(once, [&] {
@@ -146,6 +146,16 @@ int queue_kernel_per_cu(uint32_t lds) {
 #define NTT_SHAPE_INSTANCE_EXTERN(LR, LC) NTT_SHAPE_INSTANCE_ALL(extern, LR, LC)
 #define NTT_SHAPE_INSTANCE(LR, LC) NTT_SHAPE_INSTANCE_ALL(, LR, LC)
 
+// Shapes with an instance for first passes whose load forms a product or an
+// inner product (32-bit storage, matrix-core rounds, four-step post): the
+// tiles of the two-pass plans of 2^14..2^18 (pass_ip.hip).
+#define NTT_IP_SHAPES(X) X(7, 5) X(8, 4) X(8, 5) X(9, 4) X(10, 3)
+#define NTT_IP_INSTANCE_1(EXT, LR, LC)                                                                          \
+    EXT template hipError_t launch_pass_kernel<Mod32, uint32_t, uint32_t, true, LR, LC, true, true>(            \
+        const PassJob*, uint32_t, const PassJob&, uint32_t, uint32_t, hipStream_t);
+#define NTT_IP_INSTANCE_EXTERN(LR, LC) NTT_IP_INSTANCE_1(extern, LR, LC)
+#define NTT_IP_INSTANCE(LR, LC) NTT_IP_INSTANCE_1(, LR, LC)
+
 // Shapes with matrix-core-round instances for mixed storage: the first pass
 // of a transform on uint64 data (uint64 in, 32-bit out, with post) and its
 // last pass (32-bit in, uint64 out, with or without post).  These are the
@@ -173,6 +183,7 @@ NTT_PASS_INSTANCES(extern, Mod64, uint64_t, uint64_t)
 NTT_PASS_INSTANCES(extern, ModGold, uint64_t, uint64_t)
 NTT_SHAPES(NTT_SHAPE_INSTANCE_EXTERN)
 NTT_SHAPES_U64(NTT_SHAPE_U64_INSTANCE_EXTERN)
+NTT_IP_SHAPES(NTT_IP_INSTANCE_EXTERN)
 NTT_QUEUE_INSTANCES(extern)
 NTT_PIPE_SHAPES(NTT_PIPE_INSTANCE_EXTERN)
 

@@ -875,6 +875,23 @@ static bool launch_shape(uint32_t shape, const PassJob* djobs, uint32_t njobs, c
     }
 }
 
+// Shape value of the product / inner-product instances (pass_shape_of).
+constexpr uint32_t kShapeIp = 1u << 16;
+
+static bool launch_shape_ip(uint32_t shape, const PassJob* djobs, uint32_t njobs, const PassJob& inl,
+                            uint32_t nblocks, uint32_t lds, hipStream_t stream, hipError_t* e) {
+    switch (shape) {
+#define NTT_SHAPE_CASE(LR, LC)                                                                                   \
+    case (LR << 8) | LC:                                                                                          \
+        *e = launch_pass_kernel<Mod32, uint32_t, uint32_t, true, LR, LC, true, true>(djobs, njobs, inl, nblocks,  \
+                                                                                    lds, stream);                \
+        return true;
+        NTT_IP_SHAPES(NTT_SHAPE_CASE)
+#undef NTT_SHAPE_CASE
+        default: return false;
+    }
+}
+
 // One launch of the pass kernel for (arith, input storage, output storage,
 // kind = JF_MFMA / JF_POST4 bits of its jobs, all alike).
 static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, const PassJob* djobs, uint32_t njobs,
@@ -885,7 +902,10 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, co
     if (shape) {
         hipError_t e = hipSuccess;
         bool found;
-        if (sin != Store::U32 || sout != Store::U32)
+        if (shape & kShapeIp)
+            found = p4 && mf && sin == Store::U32 && sout == Store::U32 &&
+                    launch_shape_ip(shape & ~kShapeIp, djobs, njobs, inl, nblocks, lds, stream, &e);
+        else if (sin != Store::U32 || sout != Store::U32)
             found = mf && (p4 ? launch_shape_u64<true>(sin, shape, djobs, njobs, inl, nblocks, lds, stream, &e)
                               : launch_shape_u64<false>(sin, shape, djobs, njobs, inl, nblocks, lds, stream, &e));
         else
@@ -906,16 +926,28 @@ static ntt_status launch_pass(Arith ar, Store sin, Store sout, uint32_t kind, co
 // Shape-specialised kernel for job J, as (log_r << 8) | log_c, or 0: vector
 // loads and stores, full tiles, post tables in LDS, and a shape that has an
 // instance for the job's storage (32-bit in and out, or a matrix-core first
-// or last pass on uint64 data; the first may carry the fused product).
+// or last pass on uint64 data; the first may carry the fused product).  A
+// 32-bit first pass whose load forms a product or inner product runs the
+// IP instances (kShapeIp | key) where its shape has one.
 static uint32_t pass_shape_of(const PassJob& J, Arith ar, Store sin, Store sout) {
     if (ar != Arith::M32) return 0;
-    // the fused product's load: uint64 first passes only (see below), and
-    // single products only (inner products run in the generic instances)
-    if ((J.flags & JF_MUL_IN) && (sin != Store::U64 || J.ip_n > 1)) return 0;
+    const bool mul = J.flags & JF_MUL_IN;
+    // uint64 first passes take single products only
+    if (mul && sin == Store::U64 && J.ip_n > 1) return 0;
     if (!(J.flags & JF_VEC4) || !(J.flags & JF_VEC4_ST)) return 0;
     if (J.post == POST_FOURSTEP && !(J.flags & JF_POST_LDS)) return 0;
     if (J.ncols % (1u << J.log_c) != 0) return 0;
     const uint32_t key = (J.log_r << 8) | J.log_c;
+    if (mul && sin == Store::U32) {
+        if (sout != Store::U32 || !(J.flags & JF_MFMA) || !(J.flags & JF_POST4)) return 0;
+        switch (key) {
+#define NTT_SHAPE_KEY(LR, LC) case (LR << 8) | LC:
+            NTT_IP_SHAPES(NTT_SHAPE_KEY)
+#undef NTT_SHAPE_KEY
+            return kShapeIp | key;
+            default: return 0;
+        }
+    }
     if (sin == Store::U32 && sout == Store::U32) {
         switch (key) {
 #define NTT_SHAPE_KEY(LR, LC) case (LR << 8) | LC:
@@ -979,7 +1011,7 @@ static bool launch_pipe(const PassJob& J, Arith ar, Store sin, Store sout, uint3
                         hipStream_t stream, ntt_status* st) {
     const int64_t opt = g_opt_pipeline.load();
     if (opt == 0 || ar != Arith::M32 || sin != Store::U32 || sout != Store::U32 || !(J.flags & JF_MFMA) ||
-        shape == 0)
+        (J.flags & JF_MUL_IN) || shape == 0 || (shape & kShapeIp))
         return false;
     const uint32_t tiles = job_tiles(J), cus = device_cus();
     if (tiles % 8 != 0 || cus == 0) return false;

@@ -197,3 +197,41 @@ def test_ip_fused_inverse_equals_pmac_then_inverse(nt, orc):
     ctx.destroy()
     for x in (da, db, fused, sep):
         x.free()
+
+
+# The 32-bit product / inner-product instances of the first pass (one per tile
+# shape of NTT_IP_SHAPES: (7,5) (8,4) (8,5) (9,4) (10,3)), reached through
+# plans with matrix-core rounds installed for the batch's load class.
+@pytest.mark.parametrize("log_n,split,log_e", [(12, [7, 5], 12), (16, [8, 8], 12), (16, [8, 8], 13),
+                                               (18, [9, 9], 13), (20, [10, 10], 13)])
+def test_ip_shape_instances_vs_oracle(nt, orc, tmp_path, log_n, split, log_e):
+    L, dnum = 2, 3
+    n = 1 << log_n
+    q = [int(v) for v in orc.ntt_primes(1 << 31, log_n + 1, L)]
+    key = (L * n).bit_length() - 1
+    table = tmp_path / "ip_plans.txt"
+    table.write_text(f"0 {log_n} {key} 2 {split[0]} {split[1]} 0 0 {log_e} 3\n")
+    rng = np.random.default_rng(log_n * 10 + log_e)
+    a = np.stack([_residues(rng, q, n) for _ in range(dnum)])
+    b = np.stack([_residues(rng, q, n) for _ in range(dnum)])
+    flags = nt.FLAG_INVERSE | nt.FLAG_NEGACYCLIC
+    da, db = nt.DeviceBuffer.from_array(a), nt.DeviceBuffer.from_array(b)
+    dout = nt.DeviceBuffer(L * n * 4)
+    ctx = nt.Context()
+    nt.autotune_clear()
+    try:
+        nt.autotune_load(str(table))
+        bt = nt.Batch(ctx, [(dout.ptr + l * n * 4, q[l], log_n, flags) for l in range(L)],
+                      ips=[(da.ptr + l * n * 4, db.ptr + l * n * 4, dnum, L * n) for l in range(L)])
+        bt.launch()
+        nt.device_sync()
+        got = dout.download(U32, L * n).reshape(L, n).astype(np.uint64)
+        bt.destroy()
+    finally:
+        nt.autotune_clear()
+    expect = _ip_expect(orc, a, b, q, True)
+    for l in range(L):
+        assert np.array_equal(got[l], expect[l]), l
+    ctx.destroy()
+    for x in (da, db, dout):
+        x.free()
