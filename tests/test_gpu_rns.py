@@ -8,6 +8,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 U32 = np.uint32
+BB = 2013265921
 
 
 def _primes(orc, count, below=1 << 31):
@@ -234,4 +235,44 @@ def test_ip_shape_instances_vs_oracle(nt, orc, tmp_path, log_n, split, log_e):
         assert np.array_equal(got[l], expect[l]), l
     ctx.destroy()
     for x in (da, db, dout):
+        x.free()
+
+
+def test_ip_and_plain_descriptors_in_one_batch(nt, orc):
+    """Inner-product inverses (multi-pass 2^16, one-pass 2^11) and plain
+    transforms (forward 2^14, inverse 2^16 of its own data) in one batch:
+    each output equals the oracle, and the plain inputs are transformed in
+    place as in a batch without inner products."""
+    dnum = 2
+    q16 = [int(v) for v in orc.ntt_primes(1 << 31, 17, 2)]
+    rng = np.random.default_rng(99)
+    n16, n11, n14 = 1 << 16, 1 << 11, 1 << 14
+    a16 = np.stack([_residues(rng, [q16[0]], n16) for _ in range(dnum)])
+    b16 = np.stack([_residues(rng, [q16[0]], n16) for _ in range(dnum)])
+    a11 = np.stack([_residues(rng, [q16[1]], n11) for _ in range(dnum)])
+    b11 = np.stack([_residues(rng, [q16[1]], n11) for _ in range(dnum)])
+    x14 = rng.integers(0, BB, size=n14, dtype=np.uint64).astype(U32)
+    x16 = rng.integers(0, q16[1], size=n16, dtype=np.uint64).astype(U32)
+    bufs = {k: nt.DeviceBuffer.from_array(v) for k, v in
+            dict(a16=a16, b16=b16, a11=a11, b11=b11, x14=x14, x16=x16).items()}
+    o16, o11 = nt.DeviceBuffer(n16 * 4), nt.DeviceBuffer(n11 * 4)
+    inv, nega = nt.FLAG_INVERSE, nt.FLAG_NEGACYCLIC
+    ctx = nt.Context()
+    descs = [(o16.ptr, q16[0], 16, inv | nega), (bufs["x14"].ptr, BB, 14, nega), (o11.ptr, q16[1], 11, inv),
+             (bufs["x16"].ptr, q16[1], 16, inv)]
+    ips = [(bufs["a16"].ptr, bufs["b16"].ptr, dnum, n16), None, (bufs["a11"].ptr, bufs["b11"].ptr, dnum, n11), None]
+    bt = nt.Batch(ctx, descs, ips=ips)
+    bt.launch()
+    nt.device_sync()
+    e16 = _ip_expect(orc, a16, b16, [q16[0]], True)[0]
+    e11 = _ip_expect(orc, a11, b11, [q16[1]], False)[0]
+    assert np.array_equal(o16.download(U32, n16).astype(np.uint64), e16)
+    assert np.array_equal(o11.download(U32, n11).astype(np.uint64), e11)
+    assert np.array_equal(bufs["x14"].download(U32, n14).astype(np.uint64),
+                          np.asarray(orc.transform(x14.astype(np.uint64), BB, True, False), dtype=np.uint64))
+    assert np.array_equal(bufs["x16"].download(U32, n16).astype(np.uint64),
+                          np.asarray(orc.transform(x16.astype(np.uint64), q16[1], False, True), dtype=np.uint64))
+    bt.destroy()
+    ctx.destroy()
+    for x in list(bufs.values()) + [o16, o11]:
         x.free()
