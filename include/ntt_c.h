@@ -230,8 +230,8 @@ ntt_status ntt_c_batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, size
  * of canonical residues in the descriptor's storage, and its inverse
  * transform writes INTT(x) to descs[i].data: x is formed in the first pass's
  * load and never stored.  dnum = 1 is a pointwise product (the second half of
- * a negacyclic multiplication).  Needs dnum <= 64, kstride >= n when
- * dnum > 1, reserved = 0, and data not overlapping a or b; ips[i].dnum = 0
+ * a negacyclic multiplication).  Needs dnum <= 64, n <= kstride <= 2^40
+ * when dnum > 1, reserved = 0, and data not overlapping a or b; ips[i].dnum = 0
  * transforms descs[i].data as ntt_c_batch_prepare does.  The inputs are bound
  * at prepare time like the data. */
 typedef struct ntt_c_ip {

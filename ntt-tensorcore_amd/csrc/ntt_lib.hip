@@ -1836,7 +1836,7 @@ static std::vector<PassJob> merge_jobs(const std::vector<PassJob>& jobs, Store s
 
 // An inner-product source of ntt_c_batch_prepare_ip, checked before any
 // device work: an inverse transform, dnum <= 64 pairs of non-null slices
-// kstride >= n elements apart, none overlapping the descriptor's data.
+// n <= kstride <= 2^40 elements apart, none overlapping the descriptor's data.
 static ntt_status check_ip(const ntt_c_desc& d, const ntt_c_ip& ip) {
     if (ip.dnum == 0) return NTT_OK;
     if (ip.dnum > 64 || ip.reserved) return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: need dnum <= 64, reserved = 0");
@@ -1844,8 +1844,8 @@ static ntt_status check_ip(const ntt_c_desc& d, const ntt_c_ip& ip) {
         return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: an inner product feeds an inverse transform only");
     if (!ip.a || !ip.b) return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: null inner-product input");
     const int64_t n = (int64_t)1 << d.log_n;
-    if (ip.dnum > 1 && ip.kstride < n)
-        return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: slices closer than n elements (kstride < n)");
+    if (ip.dnum > 1 && (ip.kstride < n || ip.kstride > ((int64_t)1 << 40)))
+        return fail(NTT_ERR_INVALID_ARG, "batch_prepare_ip: need n <= kstride <= 2^40 elements");
     const uint64_t sb = (d.flags & NTT_FLAG_U64) ? 8 : 4;
     const uint64_t span = ((uint64_t)(ip.dnum - 1) * (uint64_t)(ip.dnum > 1 ? ip.kstride : 0) + (uint64_t)n) * sb;
     const uint64_t o = (uint64_t)(uintptr_t)d.data, ob = (uint64_t)n * sb;

@@ -95,6 +95,7 @@ def test_inner_product_batch_checked_on_the_host(nt):
         ((out, BB, 10, inv), (a, 0, 1, 0)),                # null b
         ((out, BB, 10, inv), (a, b, 65, n)),               # more than 64 pairs
         ((out, BB, 10, inv), (a, b, 2, n - 1)),            # slices overlap
+        ((out, BB, 10, inv), (a, b, 2, (1 << 40) + 1)),    # slices implausibly far apart
         ((a + 4 * n, BB, 10, inv), (a, b, 2, n)),          # data is a's second slice
         ((b - 4, BB, 10, inv), (a, b, 1, 0)),              # data runs into b
         ((a + 8 * n, BB, 10, inv | nt.FLAG_U64), (a, b, 2, n)),  # 8-byte slices: data inside a
