@@ -1881,11 +1881,12 @@ static ntt_status batch_prepare(ntt_c_context* ctx, const ntt_c_desc* descs, con
     // A batch of at least 2^24 elements that already runs a second phase
     // gives its one-pass 32-bit-arithmetic polynomials of 2^13 and 2^14 a
     // two-pass plan ([7, 6] / [7, 7], tiles of 2^13, matrix-core rounds): the
-    // second launch is paid anyway, and the split moves half of their work
-    // out of the first, longer phase.  On 512 polynomials of 2^10..2^18 the
-    // batch takes 156 us against 175 us with the one-pass plans; a batch of
-    // only such polynomials keeps the one-pass plan (2048 x 2^13: 61 us
-    // against 72 us for two passes).
+    // second launch is paid anyway, and the first phase's largest tile, which
+    // sets the LDS of every workgroup of that launch, drops from 2^14 to 2^13
+    // words (three workgroups per CU instead of two).  On 512 polynomials of
+    // 2^10..2^18 the batch takes 153 us against 176 us with the one-pass
+    // plans; a batch of only such polynomials keeps the one-pass plan
+    // (2048 x 2^13: 61 us against 72 us for two passes).
     if (two_phase && elems >= (1ull << 24))
         for (size_t i = 0; i < count; i++) {
             const uint32_t ln = descs[i].log_n;
