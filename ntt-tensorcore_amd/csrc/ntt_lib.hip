@@ -196,7 +196,10 @@ static Split heuristic_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight)
 
 // The tuned entry for (arith, log_n, elements in flight), else the tuned
 // entry for the same transform at the nearest number of elements in flight
-// within a factor of 4 (a split that won at a similar load), else the
+// within a factor of 4 (a split that won at a similar load), else, for a load
+// beyond every tuned one, the entry of the largest tuned load if that load
+// already fills the chip (2^22 elements or more: more tiles than workgroup
+// slots, so a larger batch only adds more of the same tiles), else the
 // heuristic.  (A plan tuned for a 2^24-element batch can be much slower for
 // one small transform, whose few tiles cannot fill the chip.)
 static Split choose_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight) {
@@ -206,13 +209,17 @@ static Split choose_split(uint32_t log_n, Arith ar, uint64_t elems_in_flight) {
         auto it = g_tuned.find(std::make_tuple((int)ar, log_n, le));
         if (it != g_tuned.end()) return it->second;
         const Split* best = nullptr;
+        const Split* top = nullptr;
         uint32_t best_d = 3;  // at most 2 apart in log2
+        uint32_t top_e = 0;
         for (const auto& kv : g_tuned) {
             if (std::get<0>(kv.first) != (int)ar || std::get<1>(kv.first) != log_n) continue;
             const uint32_t e = std::get<2>(kv.first), d = e > le ? e - le : le - e;
             if (d < best_d) best = &kv.second, best_d = d;
+            if (!top || e > top_e) top = &kv.second, top_e = e;
         }
         if (best) return *best;
+        if (top && le > top_e && top_e >= 22) return *top;
     }
     return heuristic_split(log_n, ar, elems_in_flight);
 }

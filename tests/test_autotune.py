@@ -41,6 +41,25 @@ def test_split_lookup_falls_back_to_nearest_batch_size(nt, tmp_path):
     assert nt.split_for(BB, 18, 1 << 15) == heur[(18, 15)]                                   # 2^20 is too far
     assert nt.split_for(BB, 16, 1 << 24) == heur[(16, 24)]                                   # 2^16 is too far
     assert nt.split_for(BB, 17, 1 << 17) == heur[(17, 17)]                                   # no entry for n
+    assert nt.split_for(BB, 18, 1 << 28) == {"split": [10, 8], "log_e": 13, "mfma_mask": 3}  # beyond 2^24
+    nt.autotune_clear()
+
+
+def test_split_lookup_beyond_the_largest_tuned_load(nt, tmp_path):
+    """A load above every tuned one takes the largest tuned load's entry when
+    that load fills the chip (2^22 elements or more), else the heuristic."""
+    nt.autotune_clear()
+    heur = {le: nt.split_for(BB, 16, 1 << le) for le in (19, 27)}
+    src = tmp_path / "t.txt"
+    src.write_text("1 22 24 2 12 10 0 0 13 0\n0 16 16 2 8 8 0 0 12 3\n")
+    nt.autotune_load(str(src))
+    Q62 = 4611686018326724609
+    tuned = {"split": [12, 10], "log_e": 13, "mfma_mask": 0}
+    assert nt.split_for(Q62, 22, 1 << 26) == tuned     # within a factor of 4
+    assert nt.split_for(Q62, 22, 1 << 27) == tuned     # beyond: the 2^24 load's plan
+    assert nt.split_for(Q62, 22, 1 << 30) == tuned
+    assert nt.split_for(BB, 16, 1 << 19) == heur[19]   # 2^16 is tuned only at a load that leaves CUs idle
+    assert nt.split_for(BB, 16, 1 << 27) == heur[27]
     nt.autotune_clear()
 
 
