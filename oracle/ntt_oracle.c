@@ -9,8 +9,8 @@
  *
  * Pinning (what checks each function independently of this text — see
  * tests/test_oracle.py):
- *   orc_find_generator / orc_create_params : BabyBear g=31 and the omega/psi
- *        values of the survey's verified table; brute-force order check.
+ *   orc_find_generator / orc_create_params : BabyBear g=31 and independently
+ *        computed omega/psi values; brute-force order check.
  *   orc_ntt_host    : O(N^2) DFT (orc_dft_naive is textbook, not a restatement)
  *                     at N<=2^10; closed-form KATs (x_j=j+1, delta, constant);
  *                     round trip; the pow-twiddle DIT of tests/test_inner_ntt.cu.
